@@ -1,0 +1,193 @@
+/*
+ * tempi_amd.h -- C ABI of libtempi_amd.so, an MI355X (gfx950) implementation
+ * of cwpearson/tempi's derived-datatype MPI_Pack / MPI_Unpack path.
+ *
+ * Two layers are exported:
+ *
+ *   1. The MPI interposition boundary.  libtempi_amd.so defines the MPI
+ *      functions below and forwards to the next definition in link order
+ *      (dlsym(RTLD_NEXT), falling back to the object that defines
+ *      MPI_Type_get_envelope), exactly as the reference does.  Link it before
+ *      libmpi or LD_PRELOAD it.
+ *
+ *   2. tempi_* entry points with plain pointers and sizes: the reference's
+ *      internal interfaces on this path (canonicalizer result, Packer,
+ *      allocators, streams), plus the batched queue, autotuner and learned
+ *      variant selector, which have no reference counterpart.
+ *
+ * Every function returns 0 (TEMPI_SUCCESS) or a negative TEMPI_ERR_* code
+ * unless stated otherwise; tempi_last_error() returns a message for the
+ * calling thread's last failure.
+ */
+#ifndef TEMPI_AMD_H
+#define TEMPI_AMD_H
+
+#include <mpi.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* 1. MPI interposition                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* reference: src/init.cpp:22 */
+int MPI_Init(int *argc, char ***argv);
+/* reference: src/init_thread.cpp:22 */
+int MPI_Init_thread(int *argc, char ***argv, int required, int *provided);
+/* reference: src/finalize.cpp:20 */
+int MPI_Finalize(void);
+/* reference: src/type_commit.cpp:16 -- canonicalizes the committed type */
+int MPI_Type_commit(MPI_Datatype *datatype);
+/* reference: src/type_free.cpp:14 -- releases the cached record first */
+int MPI_Type_free(MPI_Datatype *datatype);
+/* reference: src/pack.cpp:28 -- device/mapped buffers run the gfx950 kernel,
+   anything else goes to the underlying library.  Unlike the reference, a
+   pack that would overrun `outsize` raises MPI_ERR_TRUNCATE through the
+   communicator's error handler. */
+int MPI_Pack(const void *inbuf, int incount, MPI_Datatype datatype,
+             void *outbuf, int outsize, int *position, MPI_Comm comm);
+/* reference: src/unpack.cpp:20 -- bytes outside the type map are never
+   written */
+int MPI_Unpack(const void *inbuf, int insize, int *position, void *outbuf,
+               int outcount, MPI_Datatype datatype, MPI_Comm comm);
+
+/* ------------------------------------------------------------------------ */
+/* 2. tempi_* entry points                                                  */
+/* ------------------------------------------------------------------------ */
+
+#define TEMPI_SUCCESS 0
+#define TEMPI_ERR_ARG (-1)         /* bad argument                          */
+#define TEMPI_ERR_UNSUPPORTED (-2) /* type has no strided-block form        */
+#define TEMPI_ERR_TRUNCATE (-3)    /* output buffer too small               */
+#define TEMPI_ERR_HIP (-4)         /* HIP runtime error                     */
+#define TEMPI_ERR_NODEVICE (-5)    /* no usable GPU                         */
+#define TEMPI_ERR_NOMEM (-6)
+#define TEMPI_ERR_IO (-7)          /* model / cache file problem            */
+
+#define TEMPI_MAX_DIMS 8
+
+/* Canonical form of a committed type: `counts[0]` contiguous bytes form a
+   block, dimension d >= 1 repeats everything inside it counts[d] times,
+   strides[d] bytes apart; the first byte of object 0 sits `start` bytes
+   after the buffer origin, and object i starts i*extent bytes after object
+   0.  Blocks are listed in MPI type-map order (dimension ndims-1 slowest).
+   reference: include/strided_block.hpp:12-67 (StridedBlock) */
+typedef struct tempi_strided_block {
+  int64_t start;
+  int64_t extent;
+  int32_t ndims;
+  int32_t reserved;
+  int64_t counts[TEMPI_MAX_DIMS];
+  int64_t strides[TEMPI_MAX_DIMS];
+} tempi_strided_block;
+
+/* library version, 100*major + minor */
+int tempi_version(void);
+/* message for the calling thread's last TEMPI_ERR_* */
+const char *tempi_last_error(void);
+/* number of visible GPUs (0 when none; never fails) */
+int tempi_device_count(void);
+/* value of an MPI handle constant compiled into the library ("MPI_BYTE",
+   "MPI_FLOAT", "MPI_COMM_WORLD", ...), as an integer; `ok` is set to 0 for
+   unknown names.  Lets ABI-agnostic hosts (Python ctypes) build types. */
+int64_t tempi_mpi_constant(const char *name, int *ok);
+/* size in bytes of MPI_Datatype / MPI_Comm handles in this MPI ABI */
+int tempi_mpi_handle_size(void);
+
+/* Canonical form of a type committed through MPI_Type_commit, and its
+   packed size per object.  TEMPI_ERR_UNSUPPORTED when the type has none
+   (the MPI calls then go to the library).
+   reference: to_strided_block(simplify(traverse(t))),
+   src/internal/types.cpp:356-373, :567-614, :654-715 */
+int tempi_type_strided_block(MPI_Datatype datatype, tempi_strided_block *sb,
+                             int64_t *size);
+
+/* Canonicalize without committing through the interposer (the type must be
+   committed in the underlying library). Same output as above. */
+int tempi_type_canonicalize(MPI_Datatype datatype, tempi_strided_block *sb,
+                            int64_t *size);
+
+/* Kernel variants of the strided copy.  Variant ids are 0..n-1. */
+typedef struct tempi_variant_info {
+  int32_t lanes_per_row; /* lanes of a wavefront that share one row   */
+  int32_t word_bytes;    /* bytes per load/store                      */
+  int32_t unroll;        /* words each lane moves per work unit       */
+  int32_t nontemporal;   /* packed-side stores bypass caches          */
+} tempi_variant_info;
+int tempi_num_variants(void);
+int tempi_variant_info_get(int variant, tempi_variant_info *info);
+
+#define TEMPI_DIR_PACK 0
+#define TEMPI_DIR_UNPACK 1
+
+#define TEMPI_SELECT_HEURISTIC 0 /* closed-form rule                    */
+#define TEMPI_SELECT_MODEL 1     /* learned selector (MLP)              */
+#define TEMPI_SELECT_AUTOTUNE 2  /* exhaustive timing, cached           */
+#define TEMPI_SELECT_DEFAULT 3   /* library policy (model if loaded)    */
+
+/* Variant the library would use for this copy. */
+int tempi_select_variant(const tempi_strided_block *sb, int64_t incount,
+                         const void *strided_buf, const void *packed_buf,
+                         int direction, int method, int *variant);
+
+/* Pack `incount` objects described by `sb` from `inbuf` to outbuf +
+   *position, asynchronously on `stream` (NULL: the library stream), then
+   advance *position.  variant < 0 lets the library choose.
+   reference: Packer::pack_async, include/packer.hpp:27-35 (with the
+   synchronous Packer::pack, :38-42, being this plus a stream sync) */
+int tempi_pack_sb(const tempi_strided_block *sb, const void *inbuf,
+                  int64_t incount, void *outbuf, int64_t outsize,
+                  int64_t *position, int variant, void *stream);
+/* reference: Packer::unpack_async, include/packer.hpp:37-40 */
+int tempi_unpack_sb(const tempi_strided_block *sb, const void *inbuf,
+                    int64_t insize, int64_t *position, void *outbuf,
+                    int64_t outcount, int variant, void *stream);
+
+/* The library's non-blocking stream (reference: kernStream,
+   src/internal/streams.cpp:25), created on first use. */
+void *tempi_stream(void);
+int tempi_stream_synchronize(void *stream);
+
+/* Memory.  reference: deviceAllocator / hostAllocator
+   (include/allocator_device.hpp:22-70, include/allocator_host.hpp:31-60).
+   Mapped host memory is pinned and device-accessible: packing into it is
+   the ONE_SHOT method. */
+void *tempi_alloc_device(int64_t bytes);
+int tempi_free_device(void *p);
+void *tempi_alloc_host_mapped(int64_t bytes);
+int tempi_free_host_mapped(void *p);
+int tempi_memcpy(void *dst, const void *src, int64_t bytes); /* synchronous */
+int tempi_memset(void *dst, int value, int64_t bytes);       /* synchronous */
+/* 1 if the kernels can address `p` (device, managed or mapped host) */
+int tempi_is_device_accessible(const void *p);
+
+/* Events for device-side timing on a stream. */
+int tempi_event_create(void **event);
+int tempi_event_destroy(void *event);
+int tempi_event_record(void *event, void *stream);
+int tempi_event_synchronize(void *event);
+int tempi_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* Counters (reference: include/counters.hpp:86-193). */
+typedef struct tempi_counters {
+  int64_t pack_kernel_launches;
+  int64_t unpack_kernel_launches;
+  int64_t batch_launches;
+  int64_t graph_launches;
+  int64_t library_pack_calls;   /* MPI_Pack forwarded to the library    */
+  int64_t library_unpack_calls; /* MPI_Unpack forwarded to the library  */
+  int64_t commits_canonical;    /* MPI_Type_commit with a strided form  */
+  int64_t commits_unsupported;
+  int64_t model_inferences;     /* MLP inference kernel launches        */
+  int64_t autotune_runs;
+} tempi_counters;
+int tempi_counters_get(tempi_counters *out);
+int tempi_counters_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

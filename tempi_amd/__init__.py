@@ -1,0 +1,337 @@
+"""tempi_amd -- MI355X (gfx950) derived-datatype MPI_Pack / MPI_Unpack.
+
+The product is the C-ABI shared library ``libtempi_amd.so`` (declared in
+``include/tempi_amd.h``): it interposes MPI_Type_commit / MPI_Pack /
+MPI_Unpack in front of the system MPI and runs hand-written HIP kernels for
+device-resident buffers.  This package only loads it and gives Python hosts
+(tests, the benchmark) typed access:
+
+* :func:`lib` -- the loaded library, with ctypes prototypes;
+* :mod:`tempi_amd.mpi` -- build MPI datatypes from nested tuple specs and call
+  MPI_Pack / MPI_Unpack either through the interposer or straight into the
+  underlying MPI library;
+* :class:`DeviceBuffer` -- device or mapped host memory from the library's
+  own allocators.
+
+There is no fallback: if the library is missing, :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtempi_amd.so")
+
+TEMPI_SUCCESS = 0
+TEMPI_ERR_ARG = -1
+TEMPI_ERR_UNSUPPORTED = -2
+TEMPI_ERR_TRUNCATE = -3
+TEMPI_ERR_HIP = -4
+TEMPI_ERR_NODEVICE = -5
+TEMPI_ERR_NOMEM = -6
+TEMPI_ERR_IO = -7
+
+DIR_PACK = 0
+DIR_UNPACK = 1
+
+SELECT_HEURISTIC = 0
+SELECT_MODEL = 1
+SELECT_AUTOTUNE = 2
+SELECT_DEFAULT = 3
+
+MAX_DIMS = 8
+
+
+class StridedBlock(ctypes.Structure):
+    """tempi_strided_block (include/tempi_amd.h)."""
+
+    _fields_ = [
+        ("start", ctypes.c_int64),
+        ("extent", ctypes.c_int64),
+        ("ndims", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("counts", ctypes.c_int64 * MAX_DIMS),
+        ("strides", ctypes.c_int64 * MAX_DIMS),
+    ]
+
+    @classmethod
+    def make(cls, start, extent, counts, strides=None):
+        sb = cls()
+        sb.start = int(start)
+        sb.extent = int(extent)
+        sb.ndims = len(counts)
+        strides = list(strides) if strides is not None else [1] + [0] * (len(counts) - 1)
+        for i, c in enumerate(counts):
+            sb.counts[i] = int(c)
+            sb.strides[i] = int(strides[i])
+        return sb
+
+    def as_tuple(self):
+        n = self.ndims
+        return (self.start, self.extent, tuple(self.counts[:n]), tuple(self.strides[:n]))
+
+    def size(self) -> int:
+        s = 1
+        for i in range(self.ndims):
+            s *= self.counts[i]
+        return s
+
+    def __repr__(self):
+        st, ex, c, s = self.as_tuple()
+        return f"StridedBlock(start={st}, extent={ex}, counts={list(c)}, strides={list(s)})"
+
+
+class VariantInfo(ctypes.Structure):
+    _fields_ = [
+        ("lanes_per_row", ctypes.c_int32),
+        ("word_bytes", ctypes.c_int32),
+        ("unroll", ctypes.c_int32),
+        ("nontemporal", ctypes.c_int32),
+    ]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [
+        (n, ctypes.c_int64)
+        for n in (
+            "pack_kernel_launches",
+            "unpack_kernel_launches",
+            "batch_launches",
+            "graph_launches",
+            "library_pack_calls",
+            "library_unpack_calls",
+            "commits_canonical",
+            "commits_unsupported",
+            "model_inferences",
+            "autotune_runs",
+        )
+    ]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class TempiError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: error {code}: {last_error()}")
+        self.code = code
+
+
+_LIB = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+
+
+def _prototypes(L):
+    sb_p = ctypes.POINTER(StridedBlock)
+    i64_p = ctypes.POINTER(ctypes.c_int64)
+    protos = {
+        "tempi_version": (_I, []),
+        "tempi_last_error": (ctypes.c_char_p, []),
+        "tempi_device_count": (_I, []),
+        "tempi_mpi_constant": (_I64, [ctypes.c_char_p, ctypes.POINTER(_I)]),
+        "tempi_mpi_handle_size": (_I, []),
+        "tempi_num_variants": (_I, []),
+        "tempi_variant_info_get": (_I, [_I, ctypes.POINTER(VariantInfo)]),
+        "tempi_pack_sb": (_I, [sb_p, _P, _I64, _P, _I64, i64_p, _I, _P]),
+        "tempi_unpack_sb": (_I, [sb_p, _P, _I64, i64_p, _P, _I64, _I, _P]),
+        "tempi_select_variant": (_I, [sb_p, _I64, _P, _P, _I, _I, ctypes.POINTER(_I)]),
+        "tempi_stream": (_P, []),
+        "tempi_stream_synchronize": (_I, [_P]),
+        "tempi_alloc_device": (_P, [_I64]),
+        "tempi_free_device": (_I, [_P]),
+        "tempi_alloc_host_mapped": (_P, [_I64]),
+        "tempi_free_host_mapped": (_I, [_P]),
+        "tempi_memcpy": (_I, [_P, _P, _I64]),
+        "tempi_memset": (_I, [_P, _I, _I64]),
+        "tempi_is_device_accessible": (_I, [_P]),
+        "tempi_event_create": (_I, [ctypes.POINTER(_P)]),
+        "tempi_event_destroy": (_I, [_P]),
+        "tempi_event_record": (_I, [_P, _P]),
+        "tempi_event_synchronize": (_I, [_P]),
+        "tempi_event_elapsed_ms": (_I, [_P, _P, ctypes.POINTER(ctypes.c_float)]),
+        "tempi_counters_get": (_I, [ctypes.POINTER(Counters)]),
+        "tempi_counters_reset": (_I, []),
+        "tempi_fastdiv_check": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
+    }
+    for name, (res, args) in protos.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    # the two functions whose first argument is an MPI handle
+    h = ctypes.c_int32 if L.tempi_mpi_handle_size() == 4 else ctypes.c_int64
+    for name in ("tempi_type_strided_block", "tempi_type_canonicalize"):
+        fn = getattr(L, name)
+        fn.restype = _I
+        fn.argtypes = [h, sb_p, i64_p]
+
+
+def lib():
+    """The loaded libtempi_amd.so (loads on first call; raises if absent)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m tempi_amd.build` "
+                "(there is no non-HIP fallback)"
+            )
+        L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        _prototypes(L)
+        _LIB = L
+    return _LIB
+
+
+def last_error() -> str:
+    if _LIB is None:
+        return ""
+    s = _LIB.tempi_last_error()
+    return s.decode() if s else ""
+
+
+def check(code, what="tempi call"):
+    if code != TEMPI_SUCCESS:
+        raise TempiError(code, what)
+    return code
+
+
+def device_count() -> int:
+    return lib().tempi_device_count()
+
+
+def counters() -> dict:
+    c = Counters()
+    check(lib().tempi_counters_get(ctypes.byref(c)))
+    return c.as_dict()
+
+
+def reset_counters():
+    check(lib().tempi_counters_reset())
+
+
+def num_variants() -> int:
+    return lib().tempi_num_variants()
+
+
+def variant_info(v: int) -> dict:
+    info = VariantInfo()
+    check(lib().tempi_variant_info_get(v, ctypes.byref(info)), "variant_info")
+    return {n: getattr(info, n) for n, _ in VariantInfo._fields_}
+
+
+class DeviceBuffer:
+    """Device memory (or pinned mapped host memory with ``mapped=True``)
+    allocated by the library's allocator."""
+
+    def __init__(self, nbytes: int, mapped: bool = False):
+        L = lib()
+        self.nbytes = int(nbytes)
+        self.mapped = mapped
+        n = max(self.nbytes, 1)
+        self.ptr = L.tempi_alloc_host_mapped(n) if mapped else L.tempi_alloc_device(n)
+        if not self.ptr:
+            raise TempiError(TEMPI_ERR_NOMEM, f"allocate {n} bytes")
+
+    def free(self):
+        if self.ptr:
+            L = lib()
+            (L.tempi_free_host_mapped if self.mapped else L.tempi_free_device)(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def addr(self, offset: int = 0) -> int:
+        return int(self.ptr) + int(offset)
+
+    def upload(self, data, offset: int = 0):
+        import numpy as np
+
+        a = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+        assert offset + a.size <= self.nbytes
+        check(lib().tempi_memcpy(self.addr(offset), a.ctypes.data, a.size), "upload")
+
+    def download(self, nbytes: int | None = None, offset: int = 0):
+        import numpy as np
+
+        n = self.nbytes - offset if nbytes is None else int(nbytes)
+        out = np.empty(n, dtype=np.uint8)
+        check(lib().tempi_memcpy(out.ctypes.data, self.addr(offset), n), "download")
+        return out
+
+    def fill(self, value: int):
+        check(lib().tempi_memset(self.addr(), int(value), self.nbytes), "memset")
+
+
+class Event:
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().tempi_event_create(ctypes.byref(h)), "event_create")
+        self.h = h
+
+    def record(self, stream=None):
+        check(lib().tempi_event_record(self.h, stream), "event_record")
+
+    def synchronize(self):
+        check(lib().tempi_event_synchronize(self.h), "event_synchronize")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = ctypes.c_float()
+        check(lib().tempi_event_elapsed_ms(self.h, end.h, ctypes.byref(ms)), "elapsed")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            lib().tempi_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+def stream():
+    s = lib().tempi_stream()
+    if not s:
+        raise TempiError(TEMPI_ERR_NODEVICE, "tempi_stream")
+    return s
+
+
+def synchronize(stream_=None):
+    check(lib().tempi_stream_synchronize(stream_), "stream_synchronize")
+
+
+def pack_sb(sb: StridedBlock, inbuf: int, incount: int, outbuf: int, outsize: int,
+            position: int = 0, variant: int = -1, stream_=None) -> int:
+    """Asynchronous pack on the library stream; returns the new position."""
+    pos = ctypes.c_int64(position)
+    check(
+        lib().tempi_pack_sb(ctypes.byref(sb), inbuf, incount, outbuf, outsize,
+                            ctypes.byref(pos), variant, stream_),
+        "tempi_pack_sb",
+    )
+    return pos.value
+
+
+def unpack_sb(sb: StridedBlock, inbuf: int, insize: int, outbuf: int, outcount: int,
+              position: int = 0, variant: int = -1, stream_=None) -> int:
+    pos = ctypes.c_int64(position)
+    check(
+        lib().tempi_unpack_sb(ctypes.byref(sb), inbuf, insize, ctypes.byref(pos), outbuf,
+                              outcount, variant, stream_),
+        "tempi_unpack_sb",
+    )
+    return pos.value
+
+
+def select_variant(sb: StridedBlock, incount: int, strided: int, packed: int,
+                   direction: int = DIR_PACK, method: int = SELECT_HEURISTIC) -> int:
+    v = ctypes.c_int()
+    check(
+        lib().tempi_select_variant(ctypes.byref(sb), incount, strided, packed, direction,
+                                   method, ctypes.byref(v)),
+        "select_variant",
+    )
+    return v.value

@@ -1,0 +1,100 @@
+"""Build libtempi_amd.so in-tree with hipcc for gfx950.
+
+    python -m tempi_amd.build            # build (always recompiles)
+
+The library links the underlying MPI (MPICH by default, found through
+$MPI_HOME or /opt/conda) so that it can decode datatypes; it is meant to be
+linked before libmpi or LD_PRELOADed in front of it.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libtempi_amd.so")
+OBJDIR = os.path.join(HERE, "_build")
+
+SOURCES = [
+    "common.cpp",
+    "canonicalize.cpp",
+    "runtime.cpp",
+    "interpose.cpp",
+    "kernels.hip",
+]
+
+ARCH = os.environ.get("TEMPI_ARCH", "gfx950")
+
+
+def mpi_home() -> str:
+    for cand in (os.environ.get("MPI_HOME"), "/opt/conda"):
+        if cand and os.path.exists(os.path.join(cand, "include", "mpi.h")):
+            return cand
+    raise RuntimeError("mpi.h not found: set MPI_HOME")
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def common_flags() -> list[str]:
+    mpi = mpi_home()
+    return [
+        f"--offload-arch={ARCH}",
+        "-O3",
+        "-std=c++17",
+        "-fPIC",
+        "-fvisibility=hidden",
+        # packed fp32 instructions off for all device code
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wall",
+        "-Wno-unused-command-line-argument",
+        f"-I{os.path.join(ROOT, 'include')}",
+        f"-I{os.path.join(mpi, 'include')}",
+    ]
+
+
+def compile_one(src: str) -> str:
+    obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+    cmd = [hipcc(), *common_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+    if src.endswith(".cpp"):
+        cmd.insert(1, "-x")
+        cmd.insert(2, "hip")
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    return obj
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OBJDIR, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
+    mpi = mpi_home()
+    tmp = OUT + ".tmp"
+    cmd = [
+        hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs,
+        f"-L{os.path.join(mpi, 'lib')}", "-lmpi",
+        f"-Wl,-rpath,{os.path.join(mpi, 'lib')}", "-Wl,--enable-new-dtags",
+        "-ldl", "-o", tmp,
+    ]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, OUT)
+    if verbose:
+        print(f"built {OUT}")
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,171 @@
+// internal.hpp -- declarations shared by the translation units of
+// libtempi_amd.so.  Nothing here is part of the public ABI.
+#pragma once
+
+#include "../../include/tempi_amd.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#define TEMPI_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace tempi {
+
+// ---------------------------------------------------------------------------
+// logging and errors
+// ---------------------------------------------------------------------------
+enum { LOG_ERROR = 0, LOG_WARN = 1, LOG_INFO = 2, LOG_DEBUG = 3, LOG_SPEW = 4 };
+extern int g_log_level;
+void log_message(int level, const char *fmt, ...)
+    __attribute__((format(printf, 2, 3)));
+#define TEMPI_LOG(lvl, ...)                                                    \
+  do {                                                                         \
+    if ((lvl) <= ::tempi::g_log_level)                                         \
+      ::tempi::log_message((lvl), __VA_ARGS__);                                \
+  } while (0)
+
+// record a message for tempi_last_error() and return `code`
+int set_error(int code, const char *fmt, ...)
+    __attribute__((format(printf, 2, 3)));
+
+#define TEMPI_HIP_CHECK(expr)                                                  \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (hipSuccess != e_) {                                                    \
+      return ::tempi::set_error(TEMPI_ERR_HIP, "%s: %s (%s:%d)", #expr,        \
+                                hipGetErrorString(e_), __FILE__, __LINE__);    \
+    }                                                                          \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// environment (reference: src/internal/env.cpp:23-107, include/env.hpp)
+// ---------------------------------------------------------------------------
+struct Env {
+  bool disable = false;        // TEMPI_DISABLE: pure pass-through
+  bool no_pack = false;        // TEMPI_NO_PACK: MPI_Pack/Unpack -> library
+  bool no_type_commit = false; // TEMPI_NO_TYPE_COMMIT: no canonicalization
+  int select = TEMPI_SELECT_DEFAULT; // TEMPI_SELECT=heuristic|model|autotune
+  std::string cache_dir;       // TEMPI_CACHE_DIR (autotune cache, model)
+  std::string model_path;      // TEMPI_MODEL (MLP weights file)
+};
+Env &env();
+void read_environment();
+
+// ---------------------------------------------------------------------------
+// counters (reference: include/counters.hpp:86-193)
+// ---------------------------------------------------------------------------
+struct Counters {
+  std::atomic<int64_t> pack_kernel_launches{0};
+  std::atomic<int64_t> unpack_kernel_launches{0};
+  std::atomic<int64_t> batch_launches{0};
+  std::atomic<int64_t> graph_launches{0};
+  std::atomic<int64_t> library_pack_calls{0};
+  std::atomic<int64_t> library_unpack_calls{0};
+  std::atomic<int64_t> commits_canonical{0};
+  std::atomic<int64_t> commits_unsupported{0};
+  std::atomic<int64_t> model_inferences{0};
+  std::atomic<int64_t> autotune_runs{0};
+};
+Counters &counters();
+
+// ---------------------------------------------------------------------------
+// the underlying MPI library (reference: include/symbols.hpp:180-215,
+// src/internal/symbols.cpp:22-90)
+// ---------------------------------------------------------------------------
+struct RealMpi {
+  int (*Init)(int *, char ***) = nullptr;
+  int (*Init_thread)(int *, char ***, int, int *) = nullptr;
+  int (*Finalize)(void) = nullptr;
+  int (*Type_commit)(MPI_Datatype *) = nullptr;
+  int (*Type_free)(MPI_Datatype *) = nullptr;
+  int (*Pack)(const void *, int, MPI_Datatype, void *, int, int *,
+              MPI_Comm) = nullptr;
+  int (*Unpack)(const void *, int, int *, void *, int, MPI_Datatype,
+                MPI_Comm) = nullptr;
+};
+// resolves on first call; aborts with a message if a symbol is missing
+const RealMpi &real_mpi();
+
+// ---------------------------------------------------------------------------
+// canonical form (the commit-time lowering)
+// ---------------------------------------------------------------------------
+struct Canon {
+  bool ok = false;
+  std::string reason; // why not, when !ok
+  int64_t size = 0;   // packed bytes per object
+  int64_t lb = 0;
+  int64_t extent = 0;
+  tempi_strided_block sb{};
+};
+// decode `dt` with MPI_Type_get_envelope/contents and lower it
+Canon canonicalize(MPI_Datatype dt);
+
+// one dimension of a layout: `count` repeats `stride` bytes apart
+struct Dim {
+  int64_t count;
+  int64_t stride;
+};
+// Order-preserving normalization of a layout whose dims[0] is the byte
+// run {bytes, 1}: drop count-1 dimensions and fold a dimension into its inner
+// neighbour when it continues exactly where the inner one ends.
+void normalize_dims(std::vector<Dim> &dims);
+
+// ---------------------------------------------------------------------------
+// device runtime (reference: src/internal/streams.cpp, allocators.cpp)
+// ---------------------------------------------------------------------------
+bool have_device();           // lazily probes hipGetDeviceCount
+int ensure_runtime();         // stream etc.; TEMPI_SUCCESS or error
+hipStream_t library_stream(); // valid after ensure_runtime()
+int device_cu_count();
+// returns 1 and the device-usable address when the kernels can read/write p
+int device_address(const void *p, void **dev);
+void finalize_runtime();
+
+// ---------------------------------------------------------------------------
+// copy geometry and kernels
+// ---------------------------------------------------------------------------
+constexpr int kMaxRowDims = 8;
+
+// A pack or unpack call as rows of L bytes: row r sits at
+// strided_base + sum_k idx_k(r) * stride[k] on the strided side and at
+// packed_base + r * L on the packed side; dimension 0 is the fastest.
+struct Geom {
+  int64_t L = 0;
+  int64_t rows = 0;
+  int nrd = 0;
+  int64_t cnt[kMaxRowDims] = {};
+  int64_t str[kMaxRowDims] = {};
+  int64_t start = 0; // byte offset of row 0 from the strided buffer origin
+};
+int make_geom(const tempi_strided_block &sb, int64_t count, Geom *g);
+
+struct VariantDesc {
+  int P, W, U, NT;
+};
+extern const VariantDesc kVariants[];
+extern const int kNumVariants;
+
+// true when every row starts W-aligned on both sides and L % W == 0
+bool geom_aligned(const Geom &g, const char *strided, const char *packed,
+                  int W);
+
+// enqueue the copy; `strided` / `packed` are device addresses of the
+// buffer origin (strided) and of the first packed byte.
+int launch_copy(const Geom &g, const char *strided, char *packed, int dir,
+                int variant, hipStream_t stream, int max_blocks = 0);
+
+// closed-form variant rule (the "plain heuristic")
+int heuristic_variant(const Geom &g, const char *strided, const char *packed,
+                      int dir);
+
+// variant selection entry used by the MPI and tempi_* paths
+int choose_variant(const tempi_strided_block &sb, const Geom &g,
+                   const char *strided, const char *packed, int dir,
+                   int method);
+
+} // namespace tempi

@@ -1,0 +1,376 @@
+// interpose.cpp -- the MPI functions this library overrides, the per-type
+// record cache, and the tempi_* pack/unpack entry points.
+//
+// reference: src/init.cpp:22-67, src/init_thread.cpp:22-65,
+// src/finalize.cpp:20-46, src/type_commit.cpp:16-113, src/type_free.cpp:14-25,
+// src/pack.cpp:28-67, src/unpack.cpp:20-59, include/type_cache.hpp:23-30.
+#include "internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <unordered_map>
+
+namespace tempi {
+
+namespace {
+
+struct TypeRecord {
+  Canon canon;
+};
+
+struct TypeCache {
+  std::mutex mu;
+  std::unordered_map<int64_t, TypeRecord> map;
+};
+
+TypeCache &type_cache() {
+  static TypeCache c;
+  return c;
+}
+
+int64_t key_of(MPI_Datatype t) { return (int64_t)(intptr_t)t; }
+
+bool lookup(MPI_Datatype t, Canon *out) {
+  TypeCache &c = type_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.map.find(key_of(t));
+  if (it == c.map.end()) {
+    return false;
+  }
+  *out = it->second.canon;
+  return true;
+}
+
+std::once_flag g_env_once;
+void ensure_env() { std::call_once(g_env_once, read_environment); }
+
+// Shared body of pack and unpack on device-addressable buffers.
+// `strided` / `packed` are device addresses of the strided buffer origin and
+// of the first packed byte.
+int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
+             char *packed, int dir, int variant, hipStream_t stream) {
+  Geom g;
+  int err = make_geom(sb, count, &g);
+  if (err) {
+    return err;
+  }
+  if (g.rows == 0) {
+    return TEMPI_SUCCESS;
+  }
+  if (variant < 0) {
+    variant = choose_variant(sb, g, strided, packed, dir, env().select);
+  }
+  return launch_copy(g, strided, packed, dir, variant, stream);
+}
+
+} // namespace
+
+int choose_variant(const tempi_strided_block &sb, const Geom &g,
+                   const char *strided, const char *packed, int dir,
+                   int method) {
+  (void)sb;
+  (void)method;
+  return heuristic_variant(g, strided, packed, dir);
+}
+
+} // namespace tempi
+
+using namespace tempi;
+
+// ---------------------------------------------------------------------------
+// MPI interposition
+// ---------------------------------------------------------------------------
+
+TEMPI_EXPORT int MPI_Init(int *argc, char ***argv) {
+  ensure_env();
+  int err = real_mpi().Init(argc, argv);
+  if (env().disable || err != MPI_SUCCESS) {
+    return err;
+  }
+  // bring the stream up now so the first MPI_Pack does not pay for it
+  if (have_device()) {
+    (void)ensure_runtime();
+  }
+  return err;
+}
+
+TEMPI_EXPORT int MPI_Init_thread(int *argc, char ***argv, int required,
+                                 int *provided) {
+  ensure_env();
+  int err = real_mpi().Init_thread(argc, argv, required, provided);
+  if (env().disable || err != MPI_SUCCESS) {
+    return err;
+  }
+  if (have_device()) {
+    (void)ensure_runtime();
+  }
+  return err;
+}
+
+TEMPI_EXPORT int MPI_Finalize(void) {
+  ensure_env();
+  if (!env().disable) {
+    {
+      TypeCache &c = type_cache();
+      std::lock_guard<std::mutex> lk(c.mu);
+      c.map.clear();
+    }
+    finalize_runtime();
+  }
+  return real_mpi().Finalize();
+}
+
+TEMPI_EXPORT int MPI_Type_commit(MPI_Datatype *datatype) {
+  ensure_env();
+  int result = real_mpi().Type_commit(datatype);
+  if (env().disable || env().no_type_commit || result != MPI_SUCCESS) {
+    return result;
+  }
+  TypeRecord rec;
+  rec.canon = canonicalize(*datatype);
+  if (rec.canon.ok) {
+    ++counters().commits_canonical;
+  } else {
+    ++counters().commits_unsupported;
+    TEMPI_LOG(LOG_DEBUG, "type %lld: library path (%s)",
+              (long long)key_of(*datatype), rec.canon.reason.c_str());
+  }
+  TypeCache &c = type_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.map[key_of(*datatype)] = rec;
+  return result;
+}
+
+TEMPI_EXPORT int MPI_Type_free(MPI_Datatype *datatype) {
+  ensure_env();
+  if (!env().disable && datatype) {
+    // the library may change the handle, and may reuse its value for a new
+    // type afterwards: drop the record first (reference: type_free.cpp:20)
+    TypeCache &c = type_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.map.erase(key_of(*datatype));
+  }
+  return real_mpi().Type_free(datatype);
+}
+
+TEMPI_EXPORT int MPI_Pack(const void *inbuf, int incount, MPI_Datatype datatype,
+                          void *outbuf, int outsize, int *position,
+                          MPI_Comm comm) {
+  ensure_env();
+  const RealMpi &lib = real_mpi();
+  Canon cn;
+  void *din = nullptr, *dout = nullptr;
+  if (env().disable || env().no_pack || incount <= 0 || !position ||
+      !lookup(datatype, &cn) || !cn.ok ||
+      !device_address(inbuf, &din) || !device_address(outbuf, &dout)) {
+    ++counters().library_pack_calls;
+    return lib.Pack(inbuf, incount, datatype, outbuf, outsize, position, comm);
+  }
+  const int64_t packed = (int64_t)incount * cn.size;
+  if (*position < 0 || (int64_t)*position + packed > (int64_t)outsize) {
+    // the reference writes past the end here (pack.cpp:56)
+    MPI_Comm_call_errhandler(comm, MPI_ERR_TRUNCATE);
+    return MPI_ERR_TRUNCATE;
+  }
+  if (ensure_runtime()) {
+    ++counters().library_pack_calls;
+    return lib.Pack(inbuf, incount, datatype, outbuf, outsize, position, comm);
+  }
+  hipStream_t s = library_stream();
+  int err = run_copy(cn.sb, incount, static_cast<const char *>(din),
+                     static_cast<char *>(dout) + *position, TEMPI_DIR_PACK,
+                     -1, s);
+  if (err == TEMPI_SUCCESS && hipSuccess != hipStreamSynchronize(s)) {
+    err = set_error(TEMPI_ERR_HIP, "stream synchronize failed");
+  }
+  if (err) {
+    TEMPI_LOG(LOG_ERROR, "MPI_Pack: %s", tempi_last_error());
+    MPI_Comm_call_errhandler(comm, MPI_ERR_INTERN);
+    return MPI_ERR_INTERN;
+  }
+  *position += (int)packed;
+  return MPI_SUCCESS;
+}
+
+TEMPI_EXPORT int MPI_Unpack(const void *inbuf, int insize, int *position,
+                            void *outbuf, int outcount, MPI_Datatype datatype,
+                            MPI_Comm comm) {
+  ensure_env();
+  const RealMpi &lib = real_mpi();
+  Canon cn;
+  void *din = nullptr, *dout = nullptr;
+  if (env().disable || env().no_pack || outcount <= 0 || !position ||
+      !lookup(datatype, &cn) || !cn.ok ||
+      !device_address(inbuf, &din) || !device_address(outbuf, &dout)) {
+    ++counters().library_unpack_calls;
+    return lib.Unpack(inbuf, insize, position, outbuf, outcount, datatype,
+                      comm);
+  }
+  const int64_t packed = (int64_t)outcount * cn.size;
+  if (*position < 0 || (int64_t)*position + packed > (int64_t)insize) {
+    MPI_Comm_call_errhandler(comm, MPI_ERR_TRUNCATE);
+    return MPI_ERR_TRUNCATE;
+  }
+  if (ensure_runtime()) {
+    ++counters().library_unpack_calls;
+    return lib.Unpack(inbuf, insize, position, outbuf, outcount, datatype,
+                      comm);
+  }
+  hipStream_t s = library_stream();
+  int err = run_copy(cn.sb, outcount, static_cast<const char *>(dout),
+                     static_cast<char *>(din) + *position, TEMPI_DIR_UNPACK,
+                     -1, s);
+  if (err == TEMPI_SUCCESS && hipSuccess != hipStreamSynchronize(s)) {
+    err = set_error(TEMPI_ERR_HIP, "stream synchronize failed");
+  }
+  if (err) {
+    TEMPI_LOG(LOG_ERROR, "MPI_Unpack: %s", tempi_last_error());
+    MPI_Comm_call_errhandler(comm, MPI_ERR_INTERN);
+    return MPI_ERR_INTERN;
+  }
+  *position += (int)packed;
+  return MPI_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// tempi_* entry points
+// ---------------------------------------------------------------------------
+
+TEMPI_EXPORT int tempi_type_strided_block(MPI_Datatype datatype,
+                                          tempi_strided_block *sb,
+                                          int64_t *size) {
+  Canon cn;
+  if (!lookup(datatype, &cn)) {
+    return set_error(TEMPI_ERR_ARG,
+                     "type was not committed through MPI_Type_commit");
+  }
+  if (!cn.ok) {
+    return set_error(TEMPI_ERR_UNSUPPORTED, "%s", cn.reason.c_str());
+  }
+  if (sb) {
+    *sb = cn.sb;
+  }
+  if (size) {
+    *size = cn.size;
+  }
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_type_canonicalize(MPI_Datatype datatype,
+                                         tempi_strided_block *sb,
+                                         int64_t *size) {
+  Canon cn = canonicalize(datatype);
+  if (!cn.ok) {
+    return set_error(TEMPI_ERR_UNSUPPORTED, "%s", cn.reason.c_str());
+  }
+  if (sb) {
+    *sb = cn.sb;
+  }
+  if (size) {
+    *size = cn.size;
+  }
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_num_variants(void) { return kNumVariants; }
+
+TEMPI_EXPORT int tempi_variant_info_get(int variant, tempi_variant_info *info) {
+  if (variant < 0 || variant >= kNumVariants || !info) {
+    return set_error(TEMPI_ERR_ARG, "variant %d out of range", variant);
+  }
+  info->lanes_per_row = kVariants[variant].P;
+  info->word_bytes = kVariants[variant].W;
+  info->unroll = kVariants[variant].U;
+  info->nontemporal = kVariants[variant].NT;
+  return TEMPI_SUCCESS;
+}
+
+static int64_t sb_size(const tempi_strided_block &sb) {
+  int64_t s = 1;
+  for (int d = 0; d < sb.ndims; ++d) {
+    s *= sb.counts[d];
+  }
+  return s;
+}
+
+TEMPI_EXPORT int tempi_select_variant(const tempi_strided_block *sb,
+                                      int64_t incount, const void *strided_buf,
+                                      const void *packed_buf, int direction,
+                                      int method, int *variant) {
+  if (!sb || !variant) {
+    return set_error(TEMPI_ERR_ARG, "null argument");
+  }
+  Geom g;
+  int err = make_geom(*sb, incount, &g);
+  if (err) {
+    return err;
+  }
+  *variant = choose_variant(*sb, g, static_cast<const char *>(strided_buf),
+                            static_cast<const char *>(packed_buf), direction,
+                            method);
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_pack_sb(const tempi_strided_block *sb,
+                               const void *inbuf, int64_t incount,
+                               void *outbuf, int64_t outsize,
+                               int64_t *position, int variant, void *stream) {
+  if (!sb || !position || incount < 0 || *position < 0) {
+    return set_error(TEMPI_ERR_ARG, "bad argument");
+  }
+  const int64_t packed = incount * sb_size(*sb);
+  if (*position + packed > outsize) {
+    return set_error(TEMPI_ERR_TRUNCATE, "need %lld bytes, have %lld",
+                     (long long)(*position + packed), (long long)outsize);
+  }
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  void *din = nullptr, *dout = nullptr;
+  if (packed > 0 &&
+      (!device_address(inbuf, &din) || !device_address(outbuf, &dout))) {
+    return set_error(TEMPI_ERR_ARG, "buffers must be device-accessible");
+  }
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : library_stream();
+  err = run_copy(*sb, incount, static_cast<const char *>(din),
+                 static_cast<char *>(dout) + *position, TEMPI_DIR_PACK,
+                 variant, s);
+  if (err) {
+    return err;
+  }
+  *position += packed;
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_unpack_sb(const tempi_strided_block *sb,
+                                 const void *inbuf, int64_t insize,
+                                 int64_t *position, void *outbuf,
+                                 int64_t outcount, int variant, void *stream) {
+  if (!sb || !position || outcount < 0 || *position < 0) {
+    return set_error(TEMPI_ERR_ARG, "bad argument");
+  }
+  const int64_t packed = outcount * sb_size(*sb);
+  if (*position + packed > insize) {
+    return set_error(TEMPI_ERR_TRUNCATE, "need %lld bytes, have %lld",
+                     (long long)(*position + packed), (long long)insize);
+  }
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  void *din = nullptr, *dout = nullptr;
+  if (packed > 0 &&
+      (!device_address(inbuf, &din) || !device_address(outbuf, &dout))) {
+    return set_error(TEMPI_ERR_ARG, "buffers must be device-accessible");
+  }
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : library_stream();
+  err = run_copy(*sb, outcount, static_cast<const char *>(dout),
+                 static_cast<char *>(din) + *position, TEMPI_DIR_UNPACK,
+                 variant, s);
+  if (err) {
+    return err;
+  }
+  *position += packed;
+  return TEMPI_SUCCESS;
+}

@@ -1,0 +1,271 @@
+// runtime.cpp -- device probe, the library stream, pointer classification,
+// allocators and events.
+//
+// reference: src/internal/streams.cpp:16-42 (kernStream, non-blocking),
+// include/allocator_device.hpp / allocator_host.hpp (device memory and
+// mapped pinned host memory), src/pack.cpp:43-55 (device-accessibility test
+// with cudaPointerGetAttributes).  Everything is created lazily so that the
+// library loads, and forwards to MPI, on hosts without a GPU.
+#include "internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace tempi {
+
+namespace {
+struct Runtime {
+  std::once_flag probe_once;
+  int device_count = 0;
+  std::mutex mu;
+  bool ready = false;
+  hipStream_t stream = nullptr;
+  int cus = 0;
+};
+Runtime &rt() {
+  static Runtime r;
+  return r;
+}
+} // namespace
+
+bool have_device() {
+  Runtime &r = rt();
+  std::call_once(r.probe_once, [&r] {
+    int n = 0;
+    if (hipSuccess != hipGetDeviceCount(&n)) {
+      n = 0;
+      (void)hipGetLastError();
+    }
+    r.device_count = n;
+  });
+  return r.device_count > 0;
+}
+
+int ensure_runtime() {
+  Runtime &r = rt();
+  if (r.ready) {
+    return TEMPI_SUCCESS;
+  }
+  if (!have_device()) {
+    return set_error(TEMPI_ERR_NODEVICE, "no GPU visible to the HIP runtime");
+  }
+  std::lock_guard<std::mutex> lk(r.mu);
+  if (r.ready) {
+    return TEMPI_SUCCESS;
+  }
+  int dev = 0;
+  TEMPI_HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  TEMPI_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  r.cus = prop.multiProcessorCount;
+  // default priority, non-blocking: no implicit ordering with the caller's
+  // streams, as the reference's kernStream
+  TEMPI_HIP_CHECK(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
+  TEMPI_LOG(LOG_INFO, "device %d (%s), %d CUs", dev, prop.gcnArchName, r.cus);
+  r.ready = true;
+  return TEMPI_SUCCESS;
+}
+
+hipStream_t library_stream() { return rt().stream; }
+
+int device_cu_count() { return rt().cus > 0 ? rt().cus : 256; }
+
+int device_address(const void *p, void **dev) {
+  *dev = nullptr;
+  if (!p || !have_device()) {
+    return 0;
+  }
+  hipPointerAttribute_t a{};
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError(); // pageable host memory: not an error for us
+    return 0;
+  }
+  switch (a.type) {
+  case hipMemoryTypeDevice:
+  case hipMemoryTypeManaged:
+  case hipMemoryTypeUnified:
+    *dev = const_cast<void *>(p);
+    return 1;
+  case hipMemoryTypeHost:
+    if (!a.devicePointer) {
+      return 0;
+    }
+    if (a.hostPointer && a.hostPointer != p) {
+      // attributes describe the allocation base
+      *dev = static_cast<char *>(a.devicePointer) +
+             (static_cast<const char *>(p) -
+              static_cast<const char *>(a.hostPointer));
+    } else {
+      *dev = a.devicePointer;
+    }
+    return 1;
+  default:
+    return 0;
+  }
+}
+
+void finalize_runtime() {
+  Runtime &r = rt();
+  std::lock_guard<std::mutex> lk(r.mu);
+  if (r.ready) {
+    (void)hipStreamSynchronize(r.stream);
+    (void)hipStreamDestroy(r.stream);
+    r.stream = nullptr;
+    r.ready = false;
+  }
+}
+
+} // namespace tempi
+
+using namespace tempi;
+
+static hipStream_t as_stream(void *s) {
+  return s ? static_cast<hipStream_t>(s) : library_stream();
+}
+
+TEMPI_EXPORT int tempi_device_count(void) {
+  if (!have_device()) {
+    return 0;
+  }
+  int n = 0;
+  (void)hipGetDeviceCount(&n);
+  return n;
+}
+
+TEMPI_EXPORT void *tempi_stream(void) {
+  if (ensure_runtime() != TEMPI_SUCCESS) {
+    return nullptr;
+  }
+  return library_stream();
+}
+
+TEMPI_EXPORT int tempi_stream_synchronize(void *stream) {
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  TEMPI_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT void *tempi_alloc_device(int64_t bytes) {
+  if (bytes <= 0 || ensure_runtime() != TEMPI_SUCCESS) {
+    return nullptr;
+  }
+  void *p = nullptr;
+  if (hipSuccess != hipMalloc(&p, (size_t)bytes)) {
+    (void)hipGetLastError();
+    set_error(TEMPI_ERR_NOMEM, "hipMalloc(%lld) failed", (long long)bytes);
+    return nullptr;
+  }
+  return p;
+}
+
+TEMPI_EXPORT int tempi_free_device(void *p) {
+  if (!p) {
+    return TEMPI_SUCCESS;
+  }
+  TEMPI_HIP_CHECK(hipFree(p));
+  return TEMPI_SUCCESS;
+}
+
+// pinned, device-mapped host memory allocated as such (the target of the
+// ONE_SHOT method; the reference registers existing memory instead)
+TEMPI_EXPORT void *tempi_alloc_host_mapped(int64_t bytes) {
+  if (bytes <= 0 || ensure_runtime() != TEMPI_SUCCESS) {
+    return nullptr;
+  }
+  void *p = nullptr;
+  if (hipSuccess != hipHostMalloc(&p, (size_t)bytes,
+                                  hipHostMallocMapped | hipHostMallocPortable)) {
+    (void)hipGetLastError();
+    set_error(TEMPI_ERR_NOMEM, "hipHostMalloc(%lld) failed", (long long)bytes);
+    return nullptr;
+  }
+  return p;
+}
+
+TEMPI_EXPORT int tempi_free_host_mapped(void *p) {
+  if (!p) {
+    return TEMPI_SUCCESS;
+  }
+  TEMPI_HIP_CHECK(hipHostFree(p));
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_memcpy(void *dst, const void *src, int64_t bytes) {
+  if (bytes < 0) {
+    return set_error(TEMPI_ERR_ARG, "negative size");
+  }
+  if (bytes == 0) {
+    return TEMPI_SUCCESS;
+  }
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  TEMPI_HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDefault,
+                                 library_stream()));
+  TEMPI_HIP_CHECK(hipStreamSynchronize(library_stream()));
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_memset(void *dst, int value, int64_t bytes) {
+  if (bytes < 0) {
+    return set_error(TEMPI_ERR_ARG, "negative size");
+  }
+  if (bytes == 0) {
+    return TEMPI_SUCCESS;
+  }
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  TEMPI_HIP_CHECK(hipMemsetAsync(dst, value, (size_t)bytes, library_stream()));
+  TEMPI_HIP_CHECK(hipStreamSynchronize(library_stream()));
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_is_device_accessible(const void *p) {
+  void *d = nullptr;
+  return device_address(p, &d);
+}
+
+TEMPI_EXPORT int tempi_event_create(void **event) {
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  hipEvent_t e = nullptr;
+  TEMPI_HIP_CHECK(hipEventCreate(&e));
+  *event = e;
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_event_destroy(void *event) {
+  if (event) {
+    TEMPI_HIP_CHECK(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  }
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_event_record(void *event, void *stream) {
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  TEMPI_HIP_CHECK(
+      hipEventRecord(static_cast<hipEvent_t>(event), as_stream(stream)));
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_event_synchronize(void *event) {
+  TEMPI_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_event_elapsed_ms(void *start, void *stop, float *ms) {
+  TEMPI_HIP_CHECK(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start),
+                                      static_cast<hipEvent_t>(stop)));
+  return TEMPI_SUCCESS;
+}
