@@ -45,6 +45,20 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+def system_libdir() -> str | None:
+    """Directory the loader resolves the system libstdc++ from.  It goes
+    ahead of the MPI directory in RUNPATH, so that an older libstdc++ shipped
+    next to libmpi (conda) never shadows the one the HIP runtime needs."""
+    try:
+        out = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True).stdout
+    except OSError:
+        return None
+    for line in out.splitlines():
+        if line.strip().startswith("libstdc++.so.6 ") and "x86-64" in line:
+            return os.path.dirname(line.split("=>")[-1].strip())
+    return None
+
+
 def common_flags() -> list[str]:
     mpi = mpi_home()
     return [
@@ -80,10 +94,11 @@ def build(verbose: bool = False) -> str:
         objs = list(ex.map(compile_one, SOURCES))
     mpi = mpi_home()
     tmp = OUT + ".tmp"
+    rpath = [d for d in (system_libdir(), os.path.join(mpi, "lib")) if d]
     cmd = [
         hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs,
         f"-L{os.path.join(mpi, 'lib')}", "-lmpi",
-        f"-Wl,-rpath,{os.path.join(mpi, 'lib')}", "-Wl,--enable-new-dtags",
+        f"-Wl,-rpath,{':'.join(rpath)}", "-Wl,--enable-new-dtags",
         "-ldl", "-o", tmp,
     ]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -1,0 +1,50 @@
+import json
+import os
+import sys
+
+import pytest
+
+TESTS = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(TESTS)
+for p in (ROOT, TESTS):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and runs the HIP path")
+
+
+@pytest.fixture(scope="session")
+def mpi():
+    """The process's MPI (singleton MPI_Init through libtempi_amd.so)."""
+    from tempi_amd import mpi as tmpi
+
+    m = tmpi.get()
+    yield m
+    m.finalize()
+
+
+@pytest.fixture(scope="session")
+def golden():
+    with open(os.path.join(TESTS, "golden", "pack_golden.json")) as f:
+        return json.load(f)["cases"]
+
+
+def spec_from_json(s):
+    """JSON turns tuples into lists: restore the tuple structure of a spec."""
+    kind = s[0]
+    if kind == "named":
+        return ("named", s[1])
+    if kind == "struct":
+        return ("struct", list(s[1]), list(s[2]), [spec_from_json(c) for c in s[3]])
+    if kind == "subarray":
+        return ("subarray", list(s[1]), list(s[2]), list(s[3]), s[4], spec_from_json(s[5]))
+    if kind in ("hindexed", "indexed"):
+        return (kind, list(s[1]), list(s[2]), spec_from_json(s[3]))
+    if kind in ("indexed_block", "hindexed_block"):
+        return (kind, s[1], list(s[2]), spec_from_json(s[3]))
+    if kind == "dup":
+        return ("dup", spec_from_json(s[1]))
+    # contiguous / vector / hvector / resized: scalars then the child
+    return tuple(s[:-1]) + (spec_from_json(s[-1]),)

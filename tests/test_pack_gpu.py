@@ -1,0 +1,339 @@
+"""GPU parity of the pack/unpack path (run on an MI355X: pytest -m gpu).
+
+Every packed buffer produced by the gfx950 kernels is compared byte for byte
+with the CPU oracle and, where the case is small enough to rerun, with the
+underlying MPI's own MPI_Pack on a host copy of the same seeded bytes.
+Unpack targets are pre-filled with 0xAB and must keep that value outside the
+type map.  Calls go through the C ABI: the interposed MPI_Pack / MPI_Unpack
+and tempi_pack_sb / tempi_unpack_sb.
+"""
+import ctypes
+import hashlib
+import random
+
+import numpy as np
+import pytest
+
+import tempi_amd as ta
+import typecatalog as tc
+from conftest import spec_from_json
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    assert ta.device_count() > 0, "no GPU visible"
+
+
+def digest_ok(expected, actual: np.ndarray) -> bool:
+    if "hex" in expected:
+        return actual.tobytes().hex() == expected["hex"]
+    return hashlib.sha256(actual.tobytes()).hexdigest() == expected["sha256"]
+
+
+def mpi_roundtrip(mpi, spec, incount, seed, base_offset=0, check_mpich=True):
+    """Pack and unpack through the interposed MPI calls on device buffers;
+    returns the packed bytes."""
+    o = orc.OracleType(spec)
+    lo, hi = o.span(incount)
+    origin = -min(lo, 0) + base_offset
+    length = hi + origin + 16
+    host = orc.splitmix_bytes(length, seed)
+    expected = o.pack(host, origin, incount)
+    h = mpi.type_from_spec(spec)
+    size = expected.size
+    if check_mpich:
+        lib = np.zeros(max(size, 1), np.uint8)
+        err, pos = mpi.pack(host.ctypes.data + origin, incount, h, lib.ctypes.data, lib.size,
+                            library=True)
+        assert err == 0 and pos == size
+        assert (lib[:size] == expected).all(), "oracle disagrees with MPICH"
+    src = ta.DeviceBuffer(length)
+    src.upload(host)
+    packed = ta.DeviceBuffer(size + 64)
+    packed.fill(0x5A)
+    before = ta.counters()
+    err, pos = mpi.pack(src.addr(origin), incount, h, packed.addr(), size)
+    after = ta.counters()
+    assert err == 0 and pos == size
+    assert after["pack_kernel_launches"] > before["pack_kernel_launches"], "GPU path not taken"
+    assert after["library_pack_calls"] == before["library_pack_calls"]
+    got = packed.download(size + 64)
+    assert (got[:size] == expected).all(), f"pack mismatch for {spec}"
+    assert (got[size:] == 0x5A).all(), "pack wrote past the packed size"
+    dst = ta.DeviceBuffer(length)
+    dst.fill(0xAB)
+    err, pos = mpi.unpack(packed.addr(), size, dst.addr(origin), incount, h)
+    assert err == 0 and pos == size
+    assert ta.counters()["unpack_kernel_launches"] > after["unpack_kernel_launches"]
+    ref = np.full(length, 0xAB, np.uint8)
+    o.unpack(expected, ref, origin, incount)
+    assert (dst.download() == ref).all(), f"unpack mismatch for {spec}"
+    mpi.free(h)
+    return expected
+
+
+def test_reference_pack_unpack_cases(mpi):
+    """The case list of the reference's test/pack_unpack.cpp:160-302."""
+    mpi_roundtrip(mpi, tc.make_contiguous_contiguous(10), 30, 1)
+    for c in (1, 2):
+        mpi_roundtrip(mpi, tc.make_2d_byte_vector(2, 3, 4), c, 2)
+        mpi_roundtrip(mpi, tc.make_2d_byte_subarray(2, 3, 4), c, 3)
+        mpi_roundtrip(mpi, tc.make_off_subarray((2, 3, 4), (16, 16, 16), (1, 1, 4)), c, 4)
+        mpi_roundtrip(mpi, tc.make_byte_v_hv((2, 3, 4), (10, 10, 10)), c, 5)
+    mpi_roundtrip(mpi, tc.make_byte_v_hv((10, 10, 1), (10, 10, 10)), 1, 6)
+    mpi_roundtrip(mpi, tc.make_byte_v_hv((4, 3, 4), (200, 200, 200)), 1, 7)
+    for c in (1, 3):
+        mpi_roundtrip(mpi, tc.make_byte_v_hv((100, 100, 100), (200, 200, 200)), c, 8)
+
+
+def test_golden_fixtures(mpi, golden):
+    """Every committed MPICH fixture that lowers to a strided block, packed on
+    the GPU through MPI_Pack and unpacked into 0xAB through MPI_Unpack."""
+    n = 0
+    for case in golden:
+        spec = spec_from_json(case["spec"])
+        h = mpi.type_from_spec(spec)
+        if mpi.strided_block(h) is None:
+            mpi.free(h)
+            continue
+        L, inc, size = case["buffer_bytes"], case["incount"], case["size"] * case["incount"]
+        host = orc.splitmix_bytes(L, case["seed"])
+        src = ta.DeviceBuffer(L)
+        src.upload(host)
+        packed = ta.DeviceBuffer(max(size, 1))
+        err, pos = mpi.pack(src.addr(case["origin"]), inc, h, packed.addr(), size)
+        assert err == 0 and pos == size, case["name"]
+        assert digest_ok(case["packed"], packed.download(size)), case["name"]
+        dst = ta.DeviceBuffer(L)
+        dst.fill(0xAB)
+        err, pos = mpi.unpack(packed.addr(), size, dst.addr(case["origin"]), inc, h)
+        assert err == 0, case["name"]
+        assert digest_ok(case["unpacked_into_ab"], dst.download()), case["name"]
+        mpi.free(h)
+        n += 1
+    assert n >= 150
+
+
+SHAPES = [
+    # (block length, counts of the outer dims, strides of the outer dims)
+    (1, [64], [3]), (3, [40, 3], [7, 300]), (4, [100], [8]), (8, [33, 5], [24, 1000]),
+    (12, [64, 6], [2072, 140000]), (16, [50], [48]), (24, [17, 4], [40, 1000]),
+    (32, [30], [100]), (64, [20, 3], [64 + 40, 5000]), (100, [11], [129]),
+    (128, [25, 2], [256, 9000]), (256, [1024], [512]), (1000, [7, 3], [1500, 11000]),
+    (2048, [3, 4], [2072, 7000]), (4100, [5], [4200]), (70000, [3], [70007]),
+]
+
+
+@pytest.mark.parametrize("variant", range(12))
+def test_every_variant_every_alignment(variant):
+    """Each kernel variant, forced, on rows of 1 B - 68 KiB, with the strided
+    buffer at every byte offset 0..15 class and the packed buffer at 0 and
+    odd positions."""
+    assert ta.num_variants() == 12
+    rng = random.Random(variant)
+    for bl, counts, strides in SHAPES:
+        for base in (0, 1, 4, 8, 13):
+            for inc in (1, 2):
+                extent = strides[-1] * counts[-1] + 8
+                sb = ta.StridedBlock.make(5, extent, [bl] + counts, [1] + strides)
+                d = orc.sb_desc(*sb.as_tuple())
+                span = extent * inc + 64 + base
+                host = orc.splitmix_bytes(span, rng.randint(0, 1 << 30))
+                expected = orc.sb_pack(d, host, base, inc)
+                size = expected.size
+                pos0 = rng.choice([0, 3, 16])
+                src = ta.DeviceBuffer(span)
+                src.upload(host)
+                packed = ta.DeviceBuffer(size + pos0 + 32)
+                packed.fill(0x5A)
+                pos = ta.pack_sb(sb, src.addr(base), inc, packed.addr(), size + pos0, pos0,
+                                 variant)
+                ta.synchronize()
+                assert pos == pos0 + size
+                got = packed.download()
+                assert (got[pos0:pos0 + size] == expected).all(), (variant, bl, base, inc)
+                assert (got[:pos0] == 0x5A).all() and (got[pos0 + size:] == 0x5A).all()
+                dst = ta.DeviceBuffer(span)
+                dst.fill(0xAB)
+                pos = ta.unpack_sb(sb, packed.addr(), size + pos0, dst.addr(base), inc, pos0,
+                                   variant)
+                ta.synchronize()
+                ref = np.full(span, 0xAB, np.uint8)
+                orc.sb_unpack(d, expected, ref, base, inc)
+                assert (dst.download() == ref).all(), (variant, bl, base, inc)
+
+
+def test_headline_2d_config(mpi):
+    """vector(1024, 256, 512, MPI_BYTE), incount 1, 512 KiB source of seeded
+    random bytes (seed 1), 256-B aligned; unpack into 0xAB."""
+    spec = ("vector", 1024, 256, 512, tc.BYTE)
+    mpi_roundtrip(mpi, spec, 1, seed=1)
+    mpi_roundtrip(mpi, spec, 2, seed=1)
+
+
+def test_grid_faces_full_size(mpi):
+    """The 518^3 float32 grid (3-cell ghost ring around 512^3), normal(0,1)
+    values (seed 2): pack and unpack each face, bit-exact."""
+    n, g = 512, 3
+    full = n + 2 * g
+    values = np.random.default_rng(2).standard_normal(full ** 3, dtype=np.float32)
+    host = values.view(np.uint8)
+    src = ta.DeviceBuffer(host.size)
+    src.upload(host)
+    dst = ta.DeviceBuffer(host.size)
+    for face in "xyz":
+        spec = tc.grid_face_subarray(n, g, face)
+        h = mpi.type_from_spec(spec)
+        sb, size = mpi.strided_block(h)
+        assert size == 3 * 512 * 512 * 4
+        expected = orc.sb_pack(orc.sb_desc(*sb.as_tuple()), host, 0, 1)
+        lib = np.zeros(size, np.uint8)
+        err, pos = mpi.pack(host.ctypes.data, 1, h, lib.ctypes.data, size, library=True)
+        assert err == 0 and (lib == expected).all(), face
+        packed = ta.DeviceBuffer(size)
+        err, pos = mpi.pack(src.addr(), 1, h, packed.addr(), size)
+        assert err == 0 and pos == size
+        assert (packed.download() == expected).all(), face
+        dst.fill(0xAB)
+        err, pos = mpi.unpack(packed.addr(), size, dst.addr(), 1, h)
+        assert err == 0
+        got = dst.download()
+        mask = np.zeros(host.size, bool)
+        # the face as (z, y, x-bytes) slab of the grid
+        z0, z1 = (g, g + n) if face != "z" else (g, 2 * g)
+        y0, y1 = (g, g + n) if face != "y" else (g, 2 * g)
+        x0, x1 = (g, g + n) if face != "x" else (g, 2 * g)
+        m3 = mask.reshape(full, full, full * 4)
+        m3[z0:z1, y0:y1, 4 * x0:4 * x1] = True
+        assert (got[mask] == host[mask]).all(), face
+        assert (got[~mask] == 0xAB).all(), face
+        mpi.free(h)
+
+
+def test_alignment_shape_sweep_sample():
+    """A seeded sample of the sweep: 2D/3D hvector-of-vector byte types, block
+    length in {1..1024}, stride = bl + U{0..4096}, base offset U{0..15}."""
+    rng = random.Random(1000)
+    done = 0
+    while done < 24:
+        spec, base, packed_bytes = tc.sweep_type(rng, max_packed=8 << 20)
+        o = orc.OracleType(spec)
+        lo, hi = o.span(1)
+        if hi > (600 << 20):
+            continue
+        host = orc.splitmix_bytes(hi + base + 16, rng.randint(0, 1 << 30))
+        expected = o.pack(host, base, 1)
+        src = ta.DeviceBuffer(host.size)
+        src.upload(host)
+        out = ta.DeviceBuffer(expected.size)
+        # the strided block as the canonicalizer would give it
+        if spec[0] == "vector":
+            _, rows, bl, stride, _ = spec
+            sb = ta.StridedBlock.make(0, o.extent(), [bl, rows], [1, stride])
+        else:
+            _, planes, _, s2, (_, rows, bl, stride, _) = spec
+            sb = ta.StridedBlock.make(0, o.extent(), [bl, rows, planes], [1, stride, s2])
+        ta.pack_sb(sb, src.addr(base), 1, out.addr(), expected.size)
+        ta.synchronize()
+        assert (out.download() == expected).all(), (spec, base)
+        done += 1
+
+
+def test_one_shot_into_mapped_host_memory(mpi):
+    """MPI_Pack from device memory straight into pinned mapped host memory
+    (the ONE_SHOT method) and MPI_Unpack from it."""
+    spec = tc.grid_face_subarray(64, 3, "x")
+    o = orc.OracleType(spec)
+    host = orc.splitmix_bytes(o.extent(), 9)
+    src = ta.DeviceBuffer(o.extent())
+    src.upload(host)
+    expected = o.pack(host, 0, 1)
+    h = mpi.type_from_spec(spec)
+    mapped = ta.DeviceBuffer(expected.size, mapped=True)
+    assert ta.lib().tempi_is_device_accessible(mapped.ptr) == 1
+    before = ta.counters()["pack_kernel_launches"]
+    err, pos = mpi.pack(src.addr(), 1, h, mapped.addr(), expected.size)
+    assert err == 0 and pos == expected.size
+    assert ta.counters()["pack_kernel_launches"] == before + 1
+    view = np.ctypeslib.as_array((ctypes.c_uint8 * expected.size).from_address(mapped.ptr))
+    assert (view == expected).all()
+    dst = ta.DeviceBuffer(o.extent())
+    dst.fill(0xAB)
+    err, pos = mpi.unpack(mapped.addr(), expected.size, dst.addr(), 1, h)
+    assert err == 0
+    ref = np.full(o.extent(), 0xAB, np.uint8)
+    o.unpack(expected, ref, 0, 1)
+    assert (dst.download() == ref).all()
+    mpi.free(h)
+
+
+def test_truncation_is_an_error(mpi):
+    mpi.errors_return()
+    spec = ("vector", 16, 4, 9, tc.BYTE)
+    h = mpi.type_from_spec(spec)
+    src = ta.DeviceBuffer(256)
+    out = ta.DeviceBuffer(128)
+    out.fill(0x5A)
+    err, pos = mpi.pack(src.addr(), 1, h, out.addr(), 63)
+    assert err == mpi.ERR_TRUNCATE and pos == 0
+    err, pos = mpi.pack(src.addr(), 1, h, out.addr(), 70, position=10)
+    assert err == mpi.ERR_TRUNCATE
+    assert (out.download() == 0x5A).all()
+    err, pos = mpi.unpack(out.addr(), 63, src.addr(), 1, h)
+    assert err == mpi.ERR_TRUNCATE
+    mpi.free(h)
+
+
+def test_position_accumulates_across_types(mpi):
+    """Two different types packed back to back into one buffer, then unpacked
+    in the same order (the `position` cursor semantics)."""
+    s1 = ("vector", 7, 3, 5, tc.BYTE)
+    s2 = tc.make_byte_v_hv((5, 4, 3), (9, 8, 7))
+    o1, o2 = orc.OracleType(s1), orc.OracleType(s2)
+    h1, h2 = mpi.type_from_spec(s1), mpi.type_from_spec(s2)
+    host = orc.splitmix_bytes(4096, 77)
+    src = ta.DeviceBuffer(4096)
+    src.upload(host)
+    exp = np.concatenate([o1.pack(host, 0, 2), o2.pack(host, 100, 1)])
+    out = ta.DeviceBuffer(exp.size)
+    err, pos = mpi.pack(src.addr(), 2, h1, out.addr(), exp.size, 0)
+    assert err == 0
+    err, pos = mpi.pack(src.addr(100), 1, h2, out.addr(), exp.size, pos)
+    assert err == 0 and pos == exp.size
+    assert (out.download() == exp).all()
+    dst = ta.DeviceBuffer(4096)
+    dst.fill(0xAB)
+    err, pos = mpi.unpack(out.addr(), exp.size, dst.addr(), 2, h1, 0)
+    err, pos = mpi.unpack(out.addr(), exp.size, dst.addr(100), 1, h2, pos)
+    assert err == 0 and pos == exp.size
+    ref = np.full(4096, 0xAB, np.uint8)
+    o1.unpack(exp[: 2 * o1.size()], ref, 0, 2)
+    o2.unpack(exp[2 * o1.size():], ref, 100, 1)
+    assert (dst.download() == ref).all()
+    mpi.free(h1)
+    mpi.free(h2)
+
+
+def test_offsets_beyond_4gib():
+    """Strides past 2^32 (the reference's 32-bit `bi = y*stride1` wraps)."""
+    stride = 3 << 31
+    span = stride + 4096
+    src = ta.DeviceBuffer(span)
+    a = orc.splitmix_bytes(64, 1)
+    b = orc.splitmix_bytes(64, 2)
+    src.upload(a, 0)
+    src.upload(b, stride)
+    sb = ta.StridedBlock.make(0, span, [64, 2], [1, stride])
+    out = ta.DeviceBuffer(128)
+    ta.pack_sb(sb, src.addr(), 1, out.addr(), 128)
+    ta.synchronize()
+    assert (out.download() == np.concatenate([a, b])).all()
+    dst = ta.DeviceBuffer(span)
+    dst.fill(0xAB)
+    ta.unpack_sb(sb, out.addr(), 128, dst.addr(), 1)
+    ta.synchronize()
+    assert (dst.download(64, 0) == a).all() and (dst.download(64, stride) == b).all()
+    assert (dst.download(64, 64) == 0xAB).all() and (dst.download(64, stride - 64) == 0xAB).all()
