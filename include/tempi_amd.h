@@ -146,6 +146,29 @@ int tempi_unpack_sb(const tempi_strided_block *sb, const void *inbuf,
                     int64_t insize, int64_t *position, void *outbuf,
                     int64_t outcount, int variant, void *stream);
 
+/* Autotuner: no counterpart in the reference, which fixes the kernel by a
+   rule at commit (include/pack_kernels.cuh:129-166) and measures only whole
+   methods offline (src/internal/measure_system.cu:514-609).
+   tempi_time_variant: average device time (us) of one launch of `variant`
+   with grid cap index `grid_cap` (0..tempi_num_grid_caps()-1) over `reps`
+   back-to-back launches on the library stream.  The packed (pack) or strided
+   (unpack) buffer is written.
+   tempi_autotune: time every variant x grid cap, return the fastest and,
+   if per_variant_us is not NULL, the best time of each variant
+   (tempi_num_variants() floats).  Setting TEMPI_SELECT=autotune makes
+   MPI_Pack / MPI_Unpack tune each new shape on first use and cache it. */
+int tempi_num_grid_caps(void);
+int tempi_time_variant(const tempi_strided_block *sb, int64_t incount,
+                       const void *strided, void *packed, int direction,
+                       int variant, int grid_cap, int reps, float *us);
+int tempi_autotune(const tempi_strided_block *sb, int64_t incount,
+                   const void *strided, void *packed, int direction,
+                   int *best_variant, int *best_grid_cap,
+                   float *per_variant_us);
+/* Launch latency of an empty kernel: host-side enqueue cost and device
+   time per launch (reference: src/internal/measure_system.cu:36-64). */
+int tempi_launch_latency_us(int reps, float *host_us, float *device_us);
+
 /* The library's non-blocking stream (reference: kernStream,
    src/internal/streams.cpp:25), created on first use. */
 void *tempi_stream(void);

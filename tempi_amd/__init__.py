@@ -156,6 +156,13 @@ def _prototypes(L):
         "tempi_counters_get": (_I, [ctypes.POINTER(Counters)]),
         "tempi_counters_reset": (_I, []),
         "tempi_fastdiv_check": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
+        "tempi_num_grid_caps": (_I, []),
+        "tempi_time_variant": (_I, [sb_p, _I64, _P, _P, _I, _I, _I, _I,
+                                    ctypes.POINTER(ctypes.c_float)]),
+        "tempi_autotune": (_I, [sb_p, _I64, _P, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                ctypes.POINTER(ctypes.c_float)]),
+        "tempi_launch_latency_us": (_I, [_I, ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_float)]),
     }
     for name, (res, args) in protos.items():
         fn = getattr(L, name)
@@ -324,6 +331,29 @@ def unpack_sb(sb: StridedBlock, inbuf: int, insize: int, outbuf: int, outcount: 
         "tempi_unpack_sb",
     )
     return pos.value
+
+
+def time_variant(sb: StridedBlock, incount: int, strided: int, packed: int, direction: int,
+                 variant: int, grid_cap: int = 0, reps: int = 10) -> float:
+    us = ctypes.c_float()
+    check(lib().tempi_time_variant(ctypes.byref(sb), incount, strided, packed, direction,
+                                   variant, grid_cap, reps, ctypes.byref(us)), "time_variant")
+    return float(us.value)
+
+
+def autotune(sb: StridedBlock, incount: int, strided: int, packed: int, direction: int):
+    """(best variant, best grid cap index, best time per variant in us)."""
+    v, c = ctypes.c_int(), ctypes.c_int()
+    per = (ctypes.c_float * num_variants())()
+    check(lib().tempi_autotune(ctypes.byref(sb), incount, strided, packed, direction,
+                               ctypes.byref(v), ctypes.byref(c), per), "autotune")
+    return v.value, c.value, list(per)
+
+
+def launch_latency_us(reps: int = 1000):
+    h, d = ctypes.c_float(), ctypes.c_float()
+    check(lib().tempi_launch_latency_us(reps, ctypes.byref(h), ctypes.byref(d)), "latency")
+    return float(h.value), float(d.value)
 
 
 def select_variant(sb: StridedBlock, incount: int, strided: int, packed: int,

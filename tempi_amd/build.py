@@ -26,6 +26,7 @@ SOURCES = [
     "runtime.cpp",
     "interpose.cpp",
     "kernels.hip",
+    "autotune.hip",
 ]
 
 ARCH = os.environ.get("TEMPI_ARCH", "gfx950")

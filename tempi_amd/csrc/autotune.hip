@@ -1,0 +1,247 @@
+// autotune.hip -- exhaustive timing of kernel variants and grid caps, the
+// per-shape choice cache, and the variant selection policy.
+//
+// The reference picks its kernel with a fixed rule at commit time
+// (include/pack_kernels.cuh:129-166, src/internal/packer_3d.cu:31-69) and
+// only measures whole methods offline (src/internal/measure_system.cu:
+// 514-609, the 9x9 pack tables).  Here every (variant, grid cap) pair is
+// timed on the caller's own buffers with HIP events, the winner is cached
+// per shape, and the timings are what the learned selector is trained on.
+#include "internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <map>
+#include <sstream>
+
+namespace tempi {
+
+// multiples of the CU count; 0 = the launch code's default (16 per CU)
+const int kGridCaps[] = {0, 2, 4, 8, 32};
+const int kNumGridCaps = sizeof(kGridCaps) / sizeof(kGridCaps[0]);
+
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::string, Choice> g_tune_cache;
+
+// the shape and alignment facts a copy's speed depends on
+std::string shape_key(const Geom &g, const char *strided, const char *packed,
+                      int dir) {
+  std::ostringstream os;
+  os << dir << ':' << g.L << ':' << g.nrd;
+  for (int k = 0; k < g.nrd; ++k) {
+    os << ':' << g.cnt[k] << 'x' << g.str[k];
+  }
+  os << ":a" << ((uintptr_t)(strided + g.start) & 15) << '/'
+     << ((uintptr_t)packed & 15);
+  return os.str();
+}
+
+} // namespace
+
+int time_config(const Geom &g, const char *strided, char *packed, int dir,
+                int variant, int max_blocks, int reps, hipStream_t stream,
+                float *us) {
+  hipEvent_t a = nullptr, b = nullptr;
+  TEMPI_HIP_CHECK(hipEventCreate(&a));
+  TEMPI_HIP_CHECK(hipEventCreate(&b));
+  int err = launch_copy(g, strided, packed, dir, variant, stream, max_blocks);
+  if (!err) {
+    (void)hipEventRecord(a, stream);
+    for (int r = 0; r < reps && !err; ++r) {
+      err = launch_copy(g, strided, packed, dir, variant, stream, max_blocks);
+    }
+    (void)hipEventRecord(b, stream);
+  }
+  if (!err) {
+    float ms = 0;
+    if (hipSuccess != hipEventSynchronize(b) ||
+        hipSuccess != hipEventElapsedTime(&ms, a, b)) {
+      err = set_error(TEMPI_ERR_HIP, "event timing failed");
+    } else {
+      *us = ms * 1e3f / (float)std::max(reps, 1);
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return err;
+}
+
+int autotune(const Geom &g, const char *strided, char *packed, int dir,
+             hipStream_t stream, Choice *best, float *per_variant_us) {
+  ++counters().autotune_runs;
+  // enough launches to see past timer resolution, fewer for big copies
+  const int64_t bytes = g.L * g.rows;
+  const int reps = bytes >= (64ll << 20) ? 3 : (bytes >= (4ll << 20) ? 5 : 10);
+  float best_us = 1e30f;
+  for (int v = 0; v < kNumVariants; ++v) {
+    float vbest = 1e30f;
+    for (int c = 0; c < kNumGridCaps; ++c) {
+      const int cap = kGridCaps[c] * device_cu_count();
+      float us = 0;
+      int err = time_config(g, strided, packed, dir, v, cap, reps, stream, &us);
+      if (err) {
+        return err;
+      }
+      if (us < vbest) {
+        vbest = us;
+      }
+      if (us < best_us) {
+        best_us = us;
+        best->variant = v;
+        best->max_blocks = cap;
+      }
+    }
+    if (per_variant_us) {
+      per_variant_us[v] = vbest;
+    }
+  }
+  return TEMPI_SUCCESS;
+}
+
+Choice choose(const Geom &g, const char *strided, char *packed, int dir,
+              int method, hipStream_t stream) {
+  Choice c;
+  c.variant = heuristic_variant(g, strided, packed, dir);
+  if (method != TEMPI_SELECT_AUTOTUNE || !stream) {
+    return c;
+  }
+  const std::string key = shape_key(g, strided, packed, dir);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      return it->second;
+    }
+  }
+  Choice best = c;
+  if (autotune(g, strided, packed, dir, stream, &best, nullptr)) {
+    TEMPI_LOG(LOG_WARN, "autotune failed (%s); using the closed-form rule",
+              tempi_last_error());
+    return c;
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = best;
+  return best;
+}
+
+__global__ void k_empty(int *p) {
+  if (p) {
+    *p = threadIdx.x;
+  }
+}
+
+} // namespace tempi
+
+using namespace tempi;
+
+static int resolve_buffers(const tempi_strided_block *sb, int64_t incount,
+                           const void *strided, const void *packed, Geom *g,
+                           void **ds, void **dp) {
+  if (!sb) {
+    return set_error(TEMPI_ERR_ARG, "null strided block");
+  }
+  int err = make_geom(*sb, incount, g);
+  if (err) {
+    return err;
+  }
+  err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  if (!device_address(strided, ds) || !device_address(packed, dp)) {
+    return set_error(TEMPI_ERR_ARG, "buffers must be device-accessible");
+  }
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_num_grid_caps(void) { return kNumGridCaps; }
+
+TEMPI_EXPORT int tempi_time_variant(const tempi_strided_block *sb,
+                                    int64_t incount, const void *strided,
+                                    void *packed, int direction, int variant,
+                                    int grid_cap, int reps, float *us) {
+  Geom g;
+  void *ds = nullptr, *dp = nullptr;
+  int err = resolve_buffers(sb, incount, strided, packed, &g, &ds, &dp);
+  if (err) {
+    return err;
+  }
+  if (grid_cap < 0 || grid_cap >= kNumGridCaps) {
+    return set_error(TEMPI_ERR_ARG, "grid cap index %d", grid_cap);
+  }
+  return time_config(g, static_cast<const char *>(ds), static_cast<char *>(dp),
+                     direction, variant, kGridCaps[grid_cap] * device_cu_count(),
+                     std::max(reps, 1), library_stream(), us);
+}
+
+TEMPI_EXPORT int tempi_autotune(const tempi_strided_block *sb, int64_t incount,
+                                const void *strided, void *packed,
+                                int direction, int *best_variant,
+                                int *best_grid_cap, float *per_variant_us) {
+  Geom g;
+  void *ds = nullptr, *dp = nullptr;
+  int err = resolve_buffers(sb, incount, strided, packed, &g, &ds, &dp);
+  if (err) {
+    return err;
+  }
+  Choice best;
+  err = autotune(g, static_cast<const char *>(ds), static_cast<char *>(dp),
+                 direction, library_stream(), &best, per_variant_us);
+  if (err) {
+    return err;
+  }
+  if (best_variant) {
+    *best_variant = best.variant;
+  }
+  if (best_grid_cap) {
+    *best_grid_cap = 0;
+    for (int c = 0; c < kNumGridCaps; ++c) {
+      if (kGridCaps[c] * device_cu_count() == best.max_blocks) {
+        *best_grid_cap = c;
+      }
+    }
+  }
+  return TEMPI_SUCCESS;
+}
+
+// reference: src/internal/measure_system.cu:36-64 (empty kernel <<<1,1>>>)
+TEMPI_EXPORT int tempi_launch_latency_us(int reps, float *host_us,
+                                         float *device_us) {
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  reps = std::max(reps, 1);
+  hipStream_t s = library_stream();
+  hipEvent_t a, b;
+  TEMPI_HIP_CHECK(hipEventCreate(&a));
+  TEMPI_HIP_CHECK(hipEventCreate(&b));
+  hipLaunchKernelGGL(k_empty, dim3(1), dim3(1), 0, s, nullptr);
+  TEMPI_HIP_CHECK(hipStreamSynchronize(s));
+  TEMPI_HIP_CHECK(hipEventRecord(a, s));
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < reps; ++i) {
+    hipLaunchKernelGGL(k_empty, dim3(1), dim3(1), 0, s, nullptr);
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  TEMPI_HIP_CHECK(hipEventRecord(b, s));
+  TEMPI_HIP_CHECK(hipEventSynchronize(b));
+  float ms = 0;
+  TEMPI_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+  if (host_us) {
+    *host_us =
+        (float)std::chrono::duration<double, std::micro>(t1 - t0).count() /
+        reps;
+  }
+  if (device_us) {
+    *device_us = ms * 1e3f / reps;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return TEMPI_SUCCESS;
+}

@@ -163,9 +163,29 @@ int launch_copy(const Geom &g, const char *strided, char *packed, int dir,
 int heuristic_variant(const Geom &g, const char *strided, const char *packed,
                       int dir);
 
-// variant selection entry used by the MPI and tempi_* paths
-int choose_variant(const tempi_strided_block &sb, const Geom &g,
-                   const char *strided, const char *packed, int dir,
-                   int method);
+// a kernel variant and its grid cap (0: the default cap)
+struct Choice {
+  int variant = 0;
+  int max_blocks = 0;
+};
+
+// variant selection entry used by the MPI and tempi_* paths; AUTOTUNE times
+// the candidates on the given buffers and stream (results cached by shape)
+Choice choose(const Geom &g, const char *strided, char *packed, int dir,
+              int method, hipStream_t stream);
+
+// grid caps the autotuner tries, as multiples of the CU count (0 = default)
+extern const int kGridCaps[];
+extern const int kNumGridCaps;
+
+// average device time of one launch of (variant, max_blocks), over `reps`
+// back-to-back launches between two events on `stream`
+int time_config(const Geom &g, const char *strided, char *packed, int dir,
+                int variant, int max_blocks, int reps, hipStream_t stream,
+                float *us);
+
+// exhaustive search; fills best and, when given, the best time per variant
+int autotune(const Geom &g, const char *strided, char *packed, int dir,
+             hipStream_t stream, Choice *best, float *per_variant_us);
 
 } // namespace tempi

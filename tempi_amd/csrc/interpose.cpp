@@ -57,21 +57,16 @@ int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
   if (g.rows == 0) {
     return TEMPI_SUCCESS;
   }
+  Choice c;
   if (variant < 0) {
-    variant = choose_variant(sb, g, strided, packed, dir, env().select);
+    c = choose(g, strided, packed, dir, env().select, stream);
+  } else {
+    c.variant = variant;
   }
-  return launch_copy(g, strided, packed, dir, variant, stream);
+  return launch_copy(g, strided, packed, dir, c.variant, stream, c.max_blocks);
 }
 
 } // namespace
-
-int choose_variant(const tempi_strided_block &sb, const Geom &g,
-                   const char *strided, const char *packed, int dir,
-                   int method) {
-  (void)sb;
-  (void)method;
-  return heuristic_variant(g, strided, packed, dir);
-}
 
 } // namespace tempi
 
@@ -305,9 +300,24 @@ TEMPI_EXPORT int tempi_select_variant(const tempi_strided_block *sb,
   if (err) {
     return err;
   }
-  *variant = choose_variant(*sb, g, static_cast<const char *>(strided_buf),
-                            static_cast<const char *>(packed_buf), direction,
-                            method);
+  if (method == TEMPI_SELECT_AUTOTUNE) {
+    int err = ensure_runtime();
+    if (err) {
+      return err;
+    }
+    void *ds = nullptr, *dp = nullptr;
+    if (!device_address(strided_buf, &ds) || !device_address(packed_buf, &dp)) {
+      return set_error(TEMPI_ERR_ARG, "autotuning needs device-accessible buffers");
+    }
+    *variant = choose(g, static_cast<const char *>(ds), static_cast<char *>(dp),
+                      direction, method, library_stream())
+                   .variant;
+    return TEMPI_SUCCESS;
+  }
+  *variant = choose(g, static_cast<const char *>(strided_buf),
+                    static_cast<char *>(const_cast<void *>(packed_buf)),
+                    direction, method, nullptr)
+                 .variant;
   return TEMPI_SUCCESS;
 }
 
