@@ -1,24 +1,27 @@
 """Benchmark of the derived-datatype pack/unpack path on one MI355X.
 
-Workload (one "step"): the three 3-cell-thick faces of a 518^3 float32 grid
-(a 512^3 interior with a 3-cell ghost ring; normal(0,1) values, seed 2),
-each described by MPI_Type_create_subarray and committed through the
-interposed MPI_Type_commit; a step packs each face and unpacks it back into
-a second grid (6 kernels).  The grid stays resident in HBM.  The packs and
-unpacks are enqueued with tempi_pack_sb / tempi_unpack_sb on the library
-stream -- the kernels MPI_Pack / MPI_Unpack run, without the per-call stream
-synchronisation -- and the timed region is K steps between two device
-synchronisations.
+Headline workload (one "step"): the three 3-cell-thick faces of a 518^3
+float32 grid (a 512^3 interior with a 3-cell ghost ring; normal(0,1) values,
+seed 2), each described by MPI_Type_create_subarray and committed through the
+interposed MPI_Type_commit.  A step packs the three faces and unpacks them
+into a second grid: one batched launch for the packs and one for the unpacks
+(tempi_batch_launch on the library stream).  The grids stay resident in HBM;
+the timed region is K steps between two device synchronizations.
 
-value   = packed bytes moved per second (pack + unpack, 6 x 3 MiB per step)
-roofline: the dominant kernel's algorithmic HBM bytes per launch
-          (2 x packed bytes: read once, written once) over its average
-          duration, timed with HIP events on the library stream, against
-          the 8 TB/s HBM3E peak.
-cpu_baseline: the oracle's single-threaded C gather (oracle/oracle.c) on the
-          host copy of the same grid, over a bounded sample; the underlying
-          MPI library's own MPI_Pack (where the reference sends host buffers,
-          src/pack.cpp:51-54) is reported beside it.
+value    packed bytes moved per second, pack + unpack (6 x 3 MiB per step)
+roofline the batch kernel's algorithmic HBM bytes per launch (2 x packed:
+         each byte read once and written once) over its average device time
+         per launch, from HIP events on the library stream around the timed
+         region, against the 8 TB/s HBM3E peak
+extras   the same faces as six separate kernels (the MPI_Pack kernels) and
+         as synchronous MPI_Pack calls; cold-cache kernel times (caches
+         scrubbed by a 1 GiB write before each step); the halo batch (26
+         halos of 64 subdomains of 128^3 float64, radius 3) and 4096 random
+         requests, batched vs one call per request; the 2D vector case.
+cpu_baseline  the oracle's single-threaded C gather/scatter (oracle/oracle.c)
+         on the host copy of the grid, over a bounded sample; the underlying
+         MPI library's own MPI_Pack / MPI_Unpack (where the reference sends
+         host buffers, src/pack.cpp:51-54) is reported beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -26,7 +29,9 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import random
 import sys
 import time
 
@@ -41,6 +46,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def timed_loop(ta, fn, steps, stream):
+    """Wall and device time of `steps` calls of fn() between two syncs."""
+    a, b = ta.Event(), ta.Event()
+    ta.synchronize(stream)
+    t0 = time.perf_counter()
+    a.record(stream)
+    for _ in range(steps):
+        fn()
+    b.record(stream)
+    ta.synchronize(stream)
+    t1 = time.perf_counter()
+    return t1 - t0, a.elapsed_ms(b) * 1e-3
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,13 +67,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
-                    help="run warmup + timed steps only (for rocprofv3)")
+                    help="warmup + timed headline steps only (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share of the pack path; --gpus {args.gpus} is not "
-              "supported (the pack path does not communicate; see DESIGN.md on sharding).",
-              file=sys.stderr)
+              "supported (the pack path does not communicate; DESIGN.md describes how it "
+              "shards).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -65,6 +85,7 @@ def main():
 
     assert ta.device_count() > 0, "no GPU"
     m = tmpi.get()
+    stream = ta.stream()
     n, g = 512, 3
     full = n + 2 * g
     log("generating the 518^3 float32 grid (normal(0,1), seed 2)")
@@ -80,126 +101,133 @@ def main():
         sb, size = m.strided_block(h)
         faces.append({"face": face, "handle": h, "sb": sb, "size": size,
                       "packed": ta.DeviceBuffer(size)})
-    step_bytes = sum(2 * f["size"] for f in faces)
-    stream = ta.stream()
+    packed_per_step = sum(2 * f["size"] for f in faces)
+    pack_batch = ta.Batch([(f["sb"], grid.addr(), f["packed"].addr(), 1, ta.DIR_PACK)
+                           for f in faces])
+    unpack_batch = ta.Batch([(f["sb"], grid2.addr(), f["packed"].addr(), 1, ta.DIR_UNPACK)
+                             for f in faces])
 
-    def step():
+    def step_batched():
+        pack_batch.launch(stream)
+        unpack_batch.launch(stream)
+
+    for _ in range(args.warmup):
+        step_batched()
+    wall, dev = timed_loop(ta, step_batched, args.steps, stream)
+    ms_per_step = wall * 1e3 / args.steps
+    value = packed_per_step * args.steps / wall / 1e9
+    batch_avg_us = dev * 1e6 / (2 * args.steps)
+    log(f"batched: {ms_per_step * 1e3:.2f} us/step, {value:.1f} GB/s packed, "
+        f"batch kernel {batch_avg_us:.2f} us/launch (device)")
+    if args.profile_only:
+        print(json.dumps({"profile_only": True, "ms_per_step": ms_per_step,
+                          "batch_avg_us": batch_avg_us}))
+        return
+
+    from oracle import oracle as orc
+
+    # bit-exactness of this run against the oracle
+    bit_exact = True
+    expected = {}
+    for f in faces:
+        expected[f["face"]] = orc.sb_pack(orc.sb_desc(*f["sb"].as_tuple()), host, 0, 1)
+        bit_exact &= bool((f["packed"].download() == expected[f["face"]]).all())
+
+    extras = {}
+    # -- the same step as six single-copy kernels (what MPI_Pack launches)
+    def step_per_call():
         for f in faces:
             ta.pack_sb(f["sb"], grid.addr(), 1, f["packed"].addr(), f["size"], 0, -1, stream)
         for f in faces:
             ta.unpack_sb(f["sb"], f["packed"].addr(), f["size"], grid2.addr(), 1, 0, -1, stream)
 
     for _ in range(args.warmup):
-        step()
-    ta.synchronize(stream)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ta.synchronize(stream)
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = step_bytes * args.steps / (t1 - t0) / 1e9
-    log(f"{ms_per_step * 1e3:.2f} us/step, {value:.1f} GB/s packed")
-    if args.profile_only:
-        print(json.dumps({"profile_only": True, "ms_per_step": ms_per_step}))
-        return
+        step_per_call()
+    wall_pc, _ = timed_loop(ta, step_per_call, args.steps, stream)
+    extras["six_kernels_per_step"] = {
+        "us_per_step": round(wall_pc * 1e6 / args.steps, 3),
+        "gbs": round(packed_per_step * args.steps / wall_pc / 1e9, 1)}
+    # graph replay of the two batches
+    for _ in range(args.warmup):
+        pack_batch.graph_launch(stream)
+        unpack_batch.graph_launch(stream)
+    wall_g, _ = timed_loop(ta, lambda: (pack_batch.graph_launch(stream),
+                                        unpack_batch.graph_launch(stream)), args.steps, stream)
+    extras["batched_graph"] = {"us_per_step": round(wall_g * 1e6 / args.steps, 3),
+                               "gbs": round(packed_per_step * args.steps / wall_g / 1e9, 1)}
 
-    # per-kernel durations: HIP events around every launch on the library
-    # stream, over as many steps again
-    ops = [("pack", f) for f in faces] + [("unpack", f) for f in faces]
-    evs = [(ta.Event(), ta.Event()) for _ in ops]
-    durs = [[] for _ in ops]
-    for _ in range(args.steps):
-        for i, (kind, f) in enumerate(ops):
-            evs[i][0].record(stream)
-            if kind == "pack":
-                ta.pack_sb(f["sb"], grid.addr(), 1, f["packed"].addr(), f["size"], 0, -1, stream)
-            else:
-                ta.unpack_sb(f["sb"], f["packed"].addr(), f["size"], grid2.addr(), 1, 0, -1,
-                             stream)
-            evs[i][1].record(stream)
-        ta.synchronize(stream)
-        for i in range(len(ops)):
-            durs[i].append(evs[i][0].elapsed_ms(evs[i][1]))
+    # per-kernel device time, each single-copy kernel launched back to back
     kernels = []
-    for (kind, f), d in zip(ops, durs):
-        d = np.array(d) * 1e3  # us
-        avg = float(d.mean())
-        variant = ta.select_variant(f["sb"], 1, grid.addr(), f["packed"].addr(),
-                                    ta.DIR_PACK if kind == "pack" else ta.DIR_UNPACK)
-        kernels.append({
-            "op": f"{kind}_{f['face']}", "variant": variant,
-            "avg_us": round(avg, 3), "median_us": round(float(np.median(d)), 3),
-            "algorithmic_bytes": 2 * f["size"],
-            "hbm_gbs": round(2 * f["size"] / (avg * 1e-6) / 1e9, 1),
-        })
-    dom = max(kernels, key=lambda k: k["avg_us"])
-    log("per kernel: " + ", ".join(f"{k['op']} {k['avg_us']:.2f}us {k['hbm_gbs']:.0f}GB/s"
-                                   for k in kernels))
+    for kind in ("pack", "unpack"):
+        for f in faces:
+            def one(f=f, kind=kind):
+                if kind == "pack":
+                    ta.pack_sb(f["sb"], grid.addr(), 1, f["packed"].addr(), f["size"], 0, -1,
+                               stream)
+                else:
+                    ta.unpack_sb(f["sb"], f["packed"].addr(), f["size"], grid2.addr(), 1, 0,
+                                 -1, stream)
+            _, d = timed_loop(ta, one, 100, stream)
+            us = d * 1e6 / 100
+            kernels.append({"op": f"{kind}_{f['face']}", "us": round(us, 3),
+                            "variant": ta.select_variant(
+                                f["sb"], 1, grid.addr(), f["packed"].addr(),
+                                ta.DIR_PACK if kind == "pack" else ta.DIR_UNPACK),
+                            "hbm_gbs": round(2 * f["size"] / us / 1e3, 1)})
+    extras["single_kernels_warm"] = kernels
 
-    # bit-exactness of this run against the oracle
-    from oracle import oracle as orc
-
-    bit_exact = True
-    for f in faces:
-        exp = orc.sb_pack(orc.sb_desc(*f["sb"].as_tuple()), host, 0, 1)
-        bit_exact &= bool((f["packed"].download() == exp).all())
+    # cold caches: a 1 GiB write before each step evicts L2 and the MALL
+    scrub = ta.DeviceBuffer(1 << 30)
+    ev = [ta.Event() for _ in range(4)]
+    cold = {"pack_batch_us": [], "unpack_batch_us": []}
+    for _ in range(20):
+        scrub.fill(1)
+        ev[0].record(stream)
+        pack_batch.launch(stream)
+        ev[1].record(stream)
+        unpack_batch.launch(stream)
+        ev[2].record(stream)
+        ta.synchronize(stream)
+        cold["pack_batch_us"].append(ev[0].elapsed_ms(ev[1]) * 1e3)
+        cold["unpack_batch_us"].append(ev[1].elapsed_ms(ev[2]) * 1e3)
+    half = packed_per_step // 2
+    extras["cold_cache_batches"] = {
+        k: {"median_us": round(float(np.median(v)), 3),
+            "hbm_gbs": round(2 * half / float(np.median(v)) / 1e3, 1)}
+        for k, v in cold.items()}
+    del scrub
 
     # synchronous MPI_Pack / MPI_Unpack calls (interposed), wall clock
     call_us = {}
     for f in faces:
         ts = []
         for _ in range(50):
-            a = time.perf_counter()
+            t0 = time.perf_counter()
             err, pos = m.pack(grid.addr(), 1, f["handle"], f["packed"].addr(), f["size"])
-            ts.append(time.perf_counter() - a)
+            ts.append(time.perf_counter() - t0)
             assert err == 0
-        call_us[f["face"]] = round(float(np.median(ts)) * 1e6, 2)
+        call_us[f"MPI_Pack_{f['face']}"] = round(float(np.median(ts)) * 1e6, 2)
+        bit_exact &= bool((f["packed"].download() == expected[f["face"]]).all())
+    extras["mpi_call_wall_us"] = call_us
+
+    if not args.no_extras:
+        extras["halo_batch"] = bench_halos(ta, m, tc, stream)
+        extras["random_requests"] = bench_random_requests(ta, stream)
+        extras["vector_2d"] = bench_vector(ta, m, tc, stream)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_grid_faces_518.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            t = json.load(fh)
-        traffic = t.get(dom["op"])
+            traffic = json.load(fh).get("k_batch_per_launch")
 
-    cpu = None
-    cpu_lib = None
+    cpu = cpu_lib = None
     if not args.no_cpu:
-        cores = 1
-        log(f"cpu baseline: oracle gather, {args.cpu_seconds:.0f} s sample, 1 thread")
-        descs = [orc.sb_desc(*f["sb"].as_tuple()) for f in faces]
-        scratch = host.copy()
-        moved, t0 = 0, time.perf_counter()
-        reps = 0
-        while time.perf_counter() - t0 < args.cpu_seconds:
-            for d, f in zip(descs, faces):
-                p = orc.sb_pack(d, host, 0, 1)
-                orc.sb_unpack(d, p, scratch, 0, 1)
-                moved += 2 * f["size"]
-            reps += 1
-        el = time.perf_counter() - t0
-        cpu = {"value": round(moved / el / 1e9, 3), "unit": "GB/s", "cores": cores,
-               "kind": "port",
-               "sample": f"oracle sb_pack+sb_unpack of the 3 faces of the 518^3 float32 grid, "
-                         f"{reps} steps in {el:.1f} s on 1 host thread"}
-        log(f"cpu baseline: underlying MPI library MPI_Pack+MPI_Unpack, "
-            f"{args.cpu_seconds / 2:.0f} s sample")
-        out = np.zeros(faces[0]["size"], np.uint8)
-        moved, t0, reps = 0, time.perf_counter(), 0
-        while time.perf_counter() - t0 < args.cpu_seconds / 2:
-            for f in faces:
-                err, pos = m.pack(host.ctypes.data, 1, f["handle"], out.ctypes.data, f["size"],
-                                  library=True)
-                err, pos = m.unpack(out.ctypes.data, f["size"], scratch.ctypes.data, 1,
-                                    f["handle"], library=True)
-                moved += 2 * f["size"]
-            reps += 1
-        el = time.perf_counter() - t0
-        cpu_lib = {"value": round(moved / el / 1e9, 3), "unit": "GB/s", "cores": 1,
-                   "library": "MPICH 3.3.2 MPI_Pack/MPI_Unpack (host buffers)",
-                   "sample": f"{reps} steps of the 3 faces in {el:.1f} s"}
+        cpu, cpu_lib = cpu_baselines(args, m, orc, faces, host, np)
 
+    # each batch launch moves half the step's packed bytes, read once and
+    # written once: 2 x (packed_per_step / 2) algorithmic HBM bytes
+    achieved_gbs = packed_per_step / (batch_avg_us * 1e-6) / 1e9
     result = {
         "metric": "MPI_Pack GB/s and % HBM peak, bit-exact; batched packs/sec vs per-call plain",
         "value": round(value, 2),
@@ -215,29 +243,220 @@ def main():
         "data": "synthetic: normal(0,1) float32 grid, seed 2, resident in HBM",
         "config": {
             "workload": "3D subarray faces of a 518^3 float32 grid (512^3 + 3-cell ghost ring): "
-                        "pack+unpack of the x, y, z faces (512x512x3, 512x3x512, 3x512x512)",
-            "packed_bytes_per_step": step_bytes,
-            "path": "tempi_pack_sb/tempi_unpack_sb on the library stream (MPI_Pack's kernels)",
+                        "pack + unpack of the x, y, z faces (512x512x3, 512x3x512, 3x512x512)",
+            "packed_bytes_per_step": packed_per_step,
+            "path": "tempi_batch_launch: one persistent-queue launch packs the 3 faces, one "
+                    "unpacks them (library stream)",
         },
         "bit_exact": bit_exact,
         "roofline": {
             "bound": "hbm",
-            "kernel": dom["op"],
-            "achieved": dom["hbm_gbs"],
+            "kernel": "k_batch",
+            "achieved": round(achieved_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(dom["hbm_gbs"] / HBM_PEAK_GBS, 4),
+            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "algorithmic_bytes_per_launch": packed_per_step,
+            "avg_launch_us": round(batch_avg_us, 3),
         },
-        "kernels": kernels,
-        "mpi_pack_call_us": call_us,
+        "extras": extras,
         "cpu_baseline": cpu,
         "cpu_mpi_library": cpu_lib,
     }
     print(json.dumps(result))
     for f in faces:
         m.free(f["handle"])
+    pack_batch.destroy()
+    unpack_batch.destroy()
     m.finalize()
+
+
+def bench_halos(ta, m, tc, stream):
+    """26 halo subarrays (radius 3, float64) of 64 subdomains of 128^3:
+    one batched launch / graph replay vs one tempi_pack_sb + sync per halo."""
+    nsub, lcr, radius, quant = 64, (128, 128, 128), 3, 8
+    sub_bytes = (lcr[0] + 6) * (lcr[1] + 6) * (lcr[2] + 6) * quant
+    subs = [ta.DeviceBuffer(sub_bytes) for _ in range(nsub)]
+    for s in subs:
+        s.fill(7)
+    halos = []
+    for d in tc.halo_directions():
+        h = m.type_from_spec(tc.halo_type(radius, lcr, d, quant))
+        sb, size = m.strided_block(h)
+        m.free(h)
+        halos.append((sb, size))
+    total = nsub * sum(sz for _, sz in halos)
+    packed = ta.DeviceBuffer(total)
+    reqs, off = [], 0
+    for s in subs:
+        for sb, size in halos:
+            reqs.append((sb, s.addr(), packed.addr(off), 1, ta.DIR_PACK))
+            off += size
+    b = ta.Batch(reqs)
+    nreq = len(reqs)
+    for _ in range(5):
+        b.launch(stream)
+    reps = 50
+    wall_b, dev_b = timed_loop(ta, lambda: b.launch(stream), reps, stream)
+    wall_g, _ = timed_loop(ta, lambda: b.graph_launch(stream), reps, stream)
+
+    def per_call():
+        o = 0
+        for r in reqs:
+            ta.pack_sb(r[0], r[1], 1, packed.addr(), total, o, -1, stream)
+            ta.synchronize(stream)
+            o += r[0].size()
+
+    per_call()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        per_call()
+    wall_pc = (time.perf_counter() - t0) / 3
+
+    def per_call_async():
+        o = 0
+        for r in reqs:
+            ta.pack_sb(r[0], r[1], 1, packed.addr(), total, o, -1, stream)
+            o += r[0].size()
+
+    wall_pa, _ = timed_loop(ta, per_call_async, 3, stream)
+    wall_pa /= 3
+    res = {
+        "requests": nreq, "packed_bytes": total,
+        "batched_packs_per_s": round(nreq * reps / wall_b, 1),
+        "graph_packs_per_s": round(nreq * reps / wall_g, 1),
+        "per_call_sync_packs_per_s": round(nreq / wall_pc, 1),
+        "per_call_async_packs_per_s": round(nreq / wall_pa, 1),
+        "batched_us_per_launch": round(wall_b * 1e6 / reps, 2),
+        "batch_kernel_hbm_gbs": round(2 * total / (dev_b / reps) / 1e9, 1),
+    }
+    res["speedup_batched_vs_per_call_sync"] = round(res["batched_packs_per_s"]
+                                                    / res["per_call_sync_packs_per_s"], 1)
+    res["speedup_batched_vs_per_call_async"] = round(res["batched_packs_per_s"]
+                                                     / res["per_call_async_packs_per_s"], 1)
+    log(f"halo batch: {res}")
+    b.destroy()
+    return res
+
+
+def bench_random_requests(ta, stream, nreq=4096, seed=4):
+    """4096 random 1D/2D/3D strided blocks with packed sizes log-uniform in
+    [64 B, 16 MiB], all reading one source buffer, packed to disjoint
+    outputs: batched vs per call."""
+    rng = random.Random(seed)
+    reqs_geo = []
+    span_max = 0
+    total = 0
+    for _ in range(nreq):
+        target = int(math.exp(rng.uniform(math.log(64), math.log(16 << 20))))
+        nd = rng.choice([1, 2, 3])
+        if nd == 1:
+            counts, strides = [target], [1]
+        else:
+            bl = rng.choice([4, 8, 12, 16, 32, 64, 128, 256, 512, 1024, 2048])
+            bl = min(bl, target)
+            rows = max(1, target // bl)
+            if nd == 2:
+                counts, strides = [bl, rows], [1, bl + rng.randint(0, 512)]
+            else:
+                r1 = max(1, int(math.sqrt(rows)))
+                r2 = max(1, rows // r1)
+                s1 = bl + rng.randint(0, 512)
+                counts, strides = [bl, r1, r2], [1, s1, s1 * r1 + rng.randint(0, 4096)]
+        span = 1 + sum((c - 1) * s for c, s in zip(counts, strides)) + counts[0]
+        sb = ta.StridedBlock.make(rng.randint(0, 3) * 4, span + 16, counts, strides)
+        reqs_geo.append(sb)
+        span_max = max(span_max, span + 64)
+        total += sb.size()
+    src = ta.DeviceBuffer(span_max)
+    src.fill(3)
+    packed = ta.DeviceBuffer(total)
+    reqs, off = [], 0
+    for sb in reqs_geo:
+        reqs.append((sb, src.addr(), packed.addr(off), 1, ta.DIR_PACK))
+        off += sb.size()
+    b = ta.Batch(reqs)
+    b.launch(stream)
+    reps = 5
+    wall_b, dev_b = timed_loop(ta, lambda: b.launch(stream), reps, stream)
+
+    def per_call():
+        o = 0
+        for sb in reqs_geo:
+            ta.pack_sb(sb, src.addr(), 1, packed.addr(), total, o, -1, stream)
+            ta.synchronize(stream)
+            o += sb.size()
+
+    per_call()
+    t0 = time.perf_counter()
+    per_call()
+    wall_pc = time.perf_counter() - t0
+    res = {"requests": nreq, "packed_bytes": total,
+           "batched_packs_per_s": round(nreq * reps / wall_b, 1),
+           "per_call_sync_packs_per_s": round(nreq / wall_pc, 1),
+           "batch_kernel_hbm_gbs": round(2 * total / (dev_b / reps) / 1e9, 1)}
+    res["speedup"] = round(res["batched_packs_per_s"] / res["per_call_sync_packs_per_s"], 1)
+    log(f"random requests: {res}")
+    b.destroy()
+    return res
+
+
+def bench_vector(ta, m, tc, stream):
+    """MPI_Type_vector(1024, 256, 512, MPI_BYTE): MPI_Pack wall time and
+    kernel device time."""
+    import numpy as np
+
+    h = m.type_from_spec(("vector", 1024, 256, 512, tc.BYTE))
+    sb, size = m.strided_block(h)
+    src = ta.DeviceBuffer(524288)
+    src.fill(9)
+    out = ta.DeviceBuffer(size)
+    ts = []
+    for _ in range(200):
+        t0 = time.perf_counter()
+        m.pack(src.addr(), 1, h, out.addr(), size)
+        ts.append(time.perf_counter() - t0)
+    _, dev = timed_loop(ta, lambda: ta.pack_sb(sb, src.addr(), 1, out.addr(), size, 0, -1,
+                                               stream), 200, stream)
+    m.free(h)
+    return {"mpi_pack_wall_us": round(float(np.median(ts)) * 1e6, 2),
+            "kernel_us": round(dev * 1e6 / 200, 3),
+            "packed_bytes": size}
+
+
+def cpu_baselines(args, m, orc, faces, host, np):
+    log(f"cpu baseline: oracle gather/scatter, {args.cpu_seconds:.0f} s sample, 1 thread")
+    descs = [orc.sb_desc(*f["sb"].as_tuple()) for f in faces]
+    scratch = host.copy()
+    moved, reps = 0, 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        for d, f in zip(descs, faces):
+            p = orc.sb_pack(d, host, 0, 1)
+            orc.sb_unpack(d, p, scratch, 0, 1)
+            moved += 2 * f["size"]
+        reps += 1
+    el = time.perf_counter() - t0
+    cpu = {"value": round(moved / el / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+           "sample": f"oracle sb_pack + sb_unpack of the 3 faces of the 518^3 float32 grid, "
+                     f"{reps} steps in {el:.1f} s on 1 host thread"}
+    log(f"cpu baseline: MPI library MPI_Pack + MPI_Unpack, {args.cpu_seconds / 2:.0f} s sample")
+    out = np.zeros(faces[0]["size"], np.uint8)
+    moved, reps = 0, 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds / 2:
+        for f in faces:
+            m.pack(host.ctypes.data, 1, f["handle"], out.ctypes.data, f["size"], library=True)
+            m.unpack(out.ctypes.data, f["size"], scratch.ctypes.data, 1, f["handle"],
+                     library=True)
+            moved += 2 * f["size"]
+        reps += 1
+    el = time.perf_counter() - t0
+    lib = {"value": round(moved / el / 1e9, 3), "unit": "GB/s", "cores": 1,
+           "library": "MPICH 3.3.2 MPI_Pack / MPI_Unpack on host buffers",
+           "sample": f"{reps} steps of the 3 faces in {el:.1f} s"}
+    return cpu, lib
 
 
 if __name__ == "__main__":

@@ -169,6 +169,34 @@ int tempi_autotune(const tempi_strided_block *sb, int64_t incount,
    time per launch (reference: src/internal/measure_system.cu:36-64). */
 int tempi_launch_latency_us(int reps, float *host_us, float *device_us);
 
+/* Batched packing: no counterpart in the reference, which launches one
+   kernel per MPI_Pack and synchronizes (src/internal/packer_2d.cu:101-108).
+   A batch is a fixed set of pack and/or unpack requests (for example the 26
+   faces, edges and corners of a 3D halo), resolved once into a
+   device-resident descriptor table; one launch runs a persistent kernel that
+   drains all of them.  tempi_batch_graph_launch replays the same launch from
+   a captured hipGraph.  Requests in one batch must be independent (no
+   request writes bytes another reads or writes); launches of one batch must
+   be ordered (same stream).  Nothing is synchronized: wait on the stream. */
+typedef struct tempi_batch_request {
+  tempi_strided_block sb; /* layout of one object                          */
+  const void *strided;    /* strided buffer origin (pack: in, unpack: out)  */
+  void *packed;           /* first packed byte (pack: out, unpack: in)      */
+  int64_t count;          /* objects                                        */
+  int32_t direction;      /* TEMPI_DIR_PACK or TEMPI_DIR_UNPACK             */
+  int32_t variant;        /* kernel variant, -1 to let the library choose   */
+} tempi_batch_request;
+typedef struct tempi_batch tempi_batch;
+int tempi_batch_create(const tempi_batch_request *reqs, int64_t n, int method,
+                       tempi_batch **batch);
+int tempi_batch_launch(tempi_batch *batch, void *stream);
+int tempi_batch_graph_launch(tempi_batch *batch, void *stream);
+int tempi_batch_info(const tempi_batch *batch, int64_t *requests,
+                     int64_t *tiles, int64_t *packed_bytes);
+int tempi_batch_variant(const tempi_batch *batch, int64_t request,
+                        int *variant);
+int tempi_batch_destroy(tempi_batch *batch);
+
 /* The library's non-blocking stream (reference: kernStream,
    src/internal/streams.cpp:25), created on first use. */
 void *tempi_stream(void);

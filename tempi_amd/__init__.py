@@ -112,6 +112,19 @@ class Counters(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class BatchRequest(ctypes.Structure):
+    """tempi_batch_request (include/tempi_amd.h)."""
+
+    _fields_ = [
+        ("sb", StridedBlock),
+        ("strided", ctypes.c_void_p),
+        ("packed", ctypes.c_void_p),
+        ("count", ctypes.c_int64),
+        ("direction", ctypes.c_int32),
+        ("variant", ctypes.c_int32),
+    ]
+
+
 class TempiError(RuntimeError):
     def __init__(self, code, what):
         super().__init__(f"{what}: error {code}: {last_error()}")
@@ -163,6 +176,12 @@ def _prototypes(L):
                                 ctypes.POINTER(ctypes.c_float)]),
         "tempi_launch_latency_us": (_I, [_I, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(ctypes.c_float)]),
+        "tempi_batch_create": (_I, [ctypes.POINTER(BatchRequest), _I64, _I, ctypes.POINTER(_P)]),
+        "tempi_batch_launch": (_I, [_P, _P]),
+        "tempi_batch_graph_launch": (_I, [_P, _P]),
+        "tempi_batch_info": (_I, [_P, i64_p, i64_p, i64_p]),
+        "tempi_batch_variant": (_I, [_P, _I64, ctypes.POINTER(_I)]),
+        "tempi_batch_destroy": (_I, [_P]),
     }
     for name, (res, args) in protos.items():
         fn = getattr(L, name)
@@ -354,6 +373,56 @@ def launch_latency_us(reps: int = 1000):
     h, d = ctypes.c_float(), ctypes.c_float()
     check(lib().tempi_launch_latency_us(reps, ctypes.byref(h), ctypes.byref(d)), "latency")
     return float(h.value), float(d.value)
+
+
+class Batch:
+    """A fixed set of pack/unpack requests run by one persistent kernel.
+
+    requests: iterable of (StridedBlock, strided_addr, packed_addr, count,
+    direction[, variant])."""
+
+    def __init__(self, requests, method: int = SELECT_DEFAULT):
+        reqs = list(requests)
+        arr = (BatchRequest * max(len(reqs), 1))()
+        for i, r in enumerate(reqs):
+            sb, strided, packed, count, direction = r[:5]
+            arr[i].sb = sb
+            arr[i].strided = strided
+            arr[i].packed = packed
+            arr[i].count = count
+            arr[i].direction = direction
+            arr[i].variant = r[5] if len(r) > 5 else -1
+        h = ctypes.c_void_p()
+        check(lib().tempi_batch_create(arr, len(reqs), method, ctypes.byref(h)), "batch_create")
+        self.h = h
+        self.n = len(reqs)
+
+    def launch(self, stream_=None):
+        check(lib().tempi_batch_launch(self.h, stream_), "batch_launch")
+
+    def graph_launch(self, stream_=None):
+        check(lib().tempi_batch_graph_launch(self.h, stream_), "batch_graph_launch")
+
+    def info(self):
+        r, t, b = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().tempi_batch_info(self.h, ctypes.byref(r), ctypes.byref(t), ctypes.byref(b)))
+        return {"requests": r.value, "tiles": t.value, "packed_bytes": b.value}
+
+    def variant(self, i: int) -> int:
+        v = ctypes.c_int()
+        check(lib().tempi_batch_variant(self.h, i, ctypes.byref(v)))
+        return v.value
+
+    def destroy(self):
+        if self.h:
+            lib().tempi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def select_variant(sb: StridedBlock, incount: int, strided: int, packed: int,

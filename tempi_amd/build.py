@@ -27,6 +27,7 @@ SOURCES = [
     "interpose.cpp",
     "kernels.hip",
     "autotune.hip",
+    "batch.hip",
 ]
 
 ARCH = os.environ.get("TEMPI_ARCH", "gfx950")

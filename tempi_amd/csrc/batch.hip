@@ -1,0 +1,351 @@
+// batch.hip -- many strided copies in one persistent kernel launch.
+//
+// The reference launches one kernel per MPI_Pack and synchronizes
+// (src/internal/packer_2d.cu:101-108).  A batch instead resolves every
+// request once, on the host, into the same per-copy parameters the single
+// kernels take, uploads them as a device-resident table, and splits each
+// request into tiles of about kTileBytes.  A persistent grid (a few
+// workgroups per CU) drains the tile queue: each workgroup takes the next
+// tile index with one atomic add (fetched one tile ahead so the atomic's
+// latency overlaps the copy), finds the owning request by binary search over
+// the tiles' prefix sums, and runs that request's kernel variant on the
+// tile's units.  Heterogeneous shapes -- 12-B halo rows next to 2-KiB face
+// rows -- share one launch.
+//
+// The queue counter is reset by the last workgroup to leave, so the launch
+// needs no memset before it and can be replayed from a captured hipGraph.
+// Launches of one batch must not overlap in time (they share the counter);
+// requests inside a batch must not write bytes another request reads or
+// writes.
+#include "copy_core.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace tempi {
+
+namespace {
+
+constexpr int64_t kTileBytes = 32 << 10;
+
+struct BatchItem {
+  KParams p;
+  int32_t kind;            // 4 * variant + 2 * aligned + direction
+  uint32_t units_per_tile; // a multiple of the variant's groups per block
+};
+
+struct BatchCtl {
+  uint32_t next; // next tile to hand out
+  uint32_t done; // workgroups that have left
+};
+
+template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
+__device__ __forceinline__ void copy_range(const KParams &p, uint32_t u0,
+                                           uint32_t u1) {
+  constexpr uint32_t GROUPS = kBlock / P;
+  const uint32_t lane = threadIdx.x % P;
+  for (uint32_t u = u0 + threadIdx.x / P; u < u1; u += GROUPS) {
+    uint32_t uu = u;
+    if constexpr (P == 64) {
+      uu = __builtin_amdgcn_readfirstlane(u);
+    }
+    copy_unit<P, W, U, ALIGNED, DIR, NT>(p, uu, lane);
+  }
+}
+
+__device__ __forceinline__ void run_range(int kind, const KParams &p,
+                                          uint32_t u0, uint32_t u1) {
+  switch (kind) {
+#define TEMPI_BCASE(I, P, W, U, NT)                                            \
+  case 4 * I + 0:                                                              \
+    copy_range<P, W, U, false, 0, NT>(p, u0, u1);                              \
+    break;                                                                     \
+  case 4 * I + 1:                                                              \
+    copy_range<P, W, U, false, 1, NT>(p, u0, u1);                              \
+    break;                                                                     \
+  case 4 * I + 2:                                                              \
+    copy_range<P, W, U, true, 0, NT>(p, u0, u1);                               \
+    break;                                                                     \
+  case 4 * I + 3:                                                              \
+    copy_range<P, W, U, true, 1, NT>(p, u0, u1);                               \
+    break;
+    TEMPI_VARIANTS(TEMPI_BCASE)
+#undef TEMPI_BCASE
+  default:
+    break;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_batch(const BatchItem *__restrict__ items,
+            const uint32_t *__restrict__ tile_begin, int n, uint32_t ntiles,
+            BatchCtl *ctl) {
+  __shared__ uint32_t s_next;
+  if (threadIdx.x == 0) {
+    s_next = atomicAdd(&ctl->next, 1u);
+  }
+  __syncthreads();
+  uint32_t t = __builtin_amdgcn_readfirstlane(s_next);
+  __syncthreads();
+  while (t < ntiles) {
+    if (threadIdx.x == 0) {
+      s_next = atomicAdd(&ctl->next, 1u); // one tile ahead
+    }
+    // the request owning tile t: last r with tile_begin[r] <= t
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_begin[mid] <= t) {
+        lo = mid;
+      } else {
+        hi = mid - 1;
+      }
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const KParams p = items[lo].p;
+    const int kind = items[lo].kind;
+    const uint32_t upt = items[lo].units_per_tile;
+    const uint32_t u0 = (t - tile_begin[lo]) * upt;
+    const uint32_t u1 = min(u0 + upt, p.units);
+    run_range(kind, p, u0, u1);
+    __syncthreads();
+    t = __builtin_amdgcn_readfirstlane(s_next);
+    __syncthreads();
+  }
+  // the last workgroup out rearms the queue for the next launch
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&ctl->done, 1u) == gridDim.x - 1) {
+      atomicExch(&ctl->next, 0u);
+      atomicExch(&ctl->done, 0u);
+    }
+  }
+}
+
+} // namespace
+
+} // namespace tempi
+
+using namespace tempi;
+
+struct tempi_batch {
+  int64_t requests = 0;
+  int64_t bytes = 0;
+  uint32_t ntiles = 0;
+  int grid = 0;
+  int n = 0; // items with work
+  char *dev = nullptr; // items, tile_begin, ctl in one allocation
+  BatchItem *d_items = nullptr;
+  uint32_t *d_tile_begin = nullptr;
+  BatchCtl *d_ctl = nullptr;
+  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> variants;
+};
+
+static int batch_launch_on(tempi_batch *b, hipStream_t s) {
+  if (b->ntiles == 0) {
+    return TEMPI_SUCCESS;
+  }
+  hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0, s, b->d_items,
+                     b->d_tile_begin, b->n, b->ntiles, b->d_ctl);
+  TEMPI_HIP_CHECK(hipGetLastError());
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
+                                    int method, tempi_batch **out) {
+  if (!out || (n > 0 && !reqs) || n < 0) {
+    return set_error(TEMPI_ERR_ARG, "bad argument");
+  }
+  *out = nullptr;
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  tempi_batch *b = new tempi_batch();
+  b->requests = n;
+  std::vector<BatchItem> items;
+  std::vector<uint32_t> tile_begin;
+  uint64_t tiles = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const tempi_batch_request &r = reqs[i];
+    b->variants.push_back(-1);
+    Geom g;
+    if ((err = make_geom(r.sb, r.count, &g))) {
+      delete b;
+      return err;
+    }
+    if (g.rows == 0 || g.L == 0) {
+      continue;
+    }
+    if (g.L > (1ll << 30)) {
+      delete b;
+      return set_error(TEMPI_ERR_UNSUPPORTED,
+                       "request %lld: rows over 1 GiB are not batched",
+                       (long long)i);
+    }
+    void *ds = nullptr, *dp = nullptr;
+    if (!device_address(r.strided, &ds) || !device_address(r.packed, &dp)) {
+      delete b;
+      return set_error(TEMPI_ERR_ARG,
+                       "request %lld: buffers must be device-accessible",
+                       (long long)i);
+    }
+    const int dir = r.direction == TEMPI_DIR_UNPACK ? 1 : 0;
+    int variant = r.variant;
+    if (variant < 0 || variant >= kNumVariants) {
+      variant = choose(g, static_cast<const char *>(ds), static_cast<char *>(dp),
+                       dir, method, library_stream())
+                    .variant;
+    }
+    b->variants.back() = variant;
+    BatchItem it{};
+    bool aligned = false;
+    if ((err = fill_kparams(g, static_cast<const char *>(ds),
+                            static_cast<char *>(dp), dir, variant, &it.p,
+                            &aligned))) {
+      delete b;
+      return err;
+    }
+    it.kind = 4 * variant + 2 * (aligned ? 1 : 0) + dir;
+    const VariantDesc &v = kVariants[variant];
+    const int64_t groups = kBlock / v.P;
+    const int64_t unit_bytes =
+        std::max<int64_t>(1, std::min<int64_t>(g.L, (int64_t)v.P * v.U * v.W));
+    int64_t upt = std::max<int64_t>(groups, kTileBytes / unit_bytes);
+    upt = (upt + groups - 1) / groups * groups;
+    it.units_per_tile = (uint32_t)std::min<int64_t>(upt, 1ll << 30);
+    const uint64_t t = ((uint64_t)it.p.units + it.units_per_tile - 1) /
+                       it.units_per_tile;
+    tile_begin.push_back((uint32_t)tiles);
+    tiles += t;
+    if (tiles >= (1ull << 32)) {
+      delete b;
+      return set_error(TEMPI_ERR_UNSUPPORTED, "batch exceeds 2^32 tiles");
+    }
+    items.push_back(it);
+    b->bytes += g.L * g.rows;
+  }
+  b->n = (int)items.size();
+  b->ntiles = (uint32_t)tiles;
+  b->grid = (int)std::max<int64_t>(
+      1, std::min<int64_t>((int64_t)tiles, (int64_t)device_cu_count() * 8));
+  const size_t items_bytes = items.size() * sizeof(BatchItem);
+  const size_t tb_off = (items_bytes + 255) / 256 * 256;
+  const size_t ctl_off = tb_off + (tile_begin.size() * 4 + 255) / 256 * 256;
+  const size_t total = ctl_off + 256;
+  if (hipSuccess != hipMalloc(&b->dev, total)) {
+    (void)hipGetLastError();
+    delete b;
+    return set_error(TEMPI_ERR_NOMEM, "batch table allocation failed");
+  }
+  b->d_items = reinterpret_cast<BatchItem *>(b->dev);
+  b->d_tile_begin = reinterpret_cast<uint32_t *>(b->dev + tb_off);
+  b->d_ctl = reinterpret_cast<BatchCtl *>(b->dev + ctl_off);
+  hipStream_t s = library_stream();
+  std::vector<char> host(total, 0);
+  if (!items.empty()) {
+    memcpy(host.data(), items.data(), items_bytes);
+    memcpy(host.data() + tb_off, tile_begin.data(), tile_begin.size() * 4);
+  }
+  // one upload (the counter words are zero in `host`), outside any capture
+  if (hipSuccess != hipMemcpyAsync(b->dev, host.data(), total,
+                                   hipMemcpyHostToDevice, s) ||
+      hipSuccess != hipStreamSynchronize(s)) {
+    (void)hipGetLastError();
+    (void)hipFree(b->dev);
+    delete b;
+    return set_error(TEMPI_ERR_HIP, "batch table upload failed");
+  }
+  *out = b;
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_batch_launch(tempi_batch *b, void *stream) {
+  if (!b) {
+    return set_error(TEMPI_ERR_ARG, "null batch");
+  }
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : library_stream();
+  int err = batch_launch_on(b, s);
+  if (!err) {
+    ++counters().batch_launches;
+  }
+  return err;
+}
+
+TEMPI_EXPORT int tempi_batch_graph_launch(tempi_batch *b, void *stream) {
+  if (!b) {
+    return set_error(TEMPI_ERR_ARG, "null batch");
+  }
+  if (b->ntiles == 0) {
+    return TEMPI_SUCCESS;
+  }
+  if (!b->exec) {
+    TEMPI_HIP_CHECK(
+        hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+    TEMPI_HIP_CHECK(hipStreamBeginCapture(b->capture_stream,
+                                          hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0,
+                       b->capture_stream, b->d_items, b->d_tile_begin, b->n,
+                       b->ntiles, b->d_ctl);
+    TEMPI_HIP_CHECK(hipStreamEndCapture(b->capture_stream, &b->graph));
+    TEMPI_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : library_stream();
+  TEMPI_HIP_CHECK(hipGraphLaunch(b->exec, s));
+  ++counters().graph_launches;
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_batch_info(const tempi_batch *b, int64_t *requests,
+                                  int64_t *tiles, int64_t *packed_bytes) {
+  if (!b) {
+    return set_error(TEMPI_ERR_ARG, "null batch");
+  }
+  if (requests) {
+    *requests = b->requests;
+  }
+  if (tiles) {
+    *tiles = b->ntiles;
+  }
+  if (packed_bytes) {
+    *packed_bytes = b->bytes;
+  }
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_batch_variant(const tempi_batch *b, int64_t i,
+                                     int *variant) {
+  if (!b || i < 0 || i >= (int64_t)b->variants.size() || !variant) {
+    return set_error(TEMPI_ERR_ARG, "bad argument");
+  }
+  *variant = b->variants[i];
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_batch_destroy(tempi_batch *b) {
+  if (!b) {
+    return TEMPI_SUCCESS;
+  }
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+  }
+  if (b->capture_stream) {
+    (void)hipStreamDestroy(b->capture_stream);
+  }
+  if (b->dev) {
+    // the batch may have been launched on any stream
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b->dev);
+  }
+  delete b;
+  return TEMPI_SUCCESS;
+}

@@ -1,0 +1,193 @@
+// copy_core.hpp -- the per-unit strided copy shared by the single-copy
+// kernels (kernels.hip) and the batched queue kernel (batch.hip).
+//
+// A copy is `units` work units; unit u is segment (u % nseg) of row
+// (u / nseg).  A group of P lanes moves one unit as U words of W bytes per
+// lane, the P lanes touching consecutive words.  Stores are always aligned
+// to the word's natural unit A (W for powers of two, 4 for 12-byte words);
+// loads may sit at any byte address (gfx9 global loads accept that).
+#pragma once
+
+#include "internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace tempi {
+
+// n / d for 32-bit n by multiply-high and shift (Granlund and Montgomery,
+// "Division by Invariant Integers using Multiplication", PLDI 1994, fig. 4.1):
+// with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (mulhi(n, m) + n) >> l, the sum taken in 64 bits.
+struct FastDiv {
+  uint32_t mul = 1;
+  uint32_t shift = 0;
+  __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    const uint32_t t = (uint32_t)(((uint64_t)n * mul) >> 32);
+    return (uint32_t)(((uint64_t)t + n) >> shift);
+  }
+};
+
+FastDiv make_fastdiv(uint32_t d);
+
+struct KParams {
+  const char *src; // first byte of row 0 on the source side
+  char *dst;       // first byte of row 0 on the destination side
+  int64_t L;       // bytes per row
+  uint32_t units;  // rows * nseg
+  uint32_t nseg;   // segments per row
+  FastDiv segdiv;
+  uint32_t words; // L / W
+  int32_t nrd;    // row dimensions
+  uint32_t cnt[kMaxRowDims];
+  FastDiv div[kMaxRowDims];
+  int64_t str[kMaxRowDims];
+};
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <int W> struct WordT;
+template <> struct WordT<16> { typedef u32x4 T; };
+template <> struct WordT<12> { typedef u32x3 T; };
+template <> struct WordT<8> { typedef u32x2 T; };
+template <> struct WordT<4> { typedef uint32_t T; };
+template <> struct WordT<2> { typedef uint16_t T; };
+template <> struct WordT<1> { typedef uint8_t T; };
+
+// alignment unit of a W-byte word
+__host__ __device__ constexpr int word_align(int W) {
+  return (W & (W - 1)) == 0 ? W : 4;
+}
+
+// load W bytes from any byte address
+template <int W>
+__device__ __forceinline__ typename WordT<W>::T load_word(const char *p) {
+  typedef typename WordT<W>::T T;
+  typedef T Tu __attribute__((aligned(1)));
+  return *reinterpret_cast<const Tu *>(p);
+}
+
+// store W bytes to an address aligned to word_align(W)
+template <int W, bool NT>
+__device__ __forceinline__ void store_word(char *p, typename WordT<W>::T v) {
+  typedef typename WordT<W>::T T;
+  typedef T Ta __attribute__((aligned(word_align(W))));
+  if constexpr (NT) {
+    __builtin_nontemporal_store(v, reinterpret_cast<Ta *>(p));
+  } else {
+    *reinterpret_cast<Ta *>(p) = v;
+  }
+}
+
+// byte offset of row `row` on the strided side
+__device__ __forceinline__ int64_t row_offset(const KParams &p, uint32_t row) {
+  int64_t off = 0;
+  uint32_t q = row;
+#pragma unroll
+  for (int k = 0; k < kMaxRowDims; ++k) {
+    if (k >= p.nrd) {
+      break;
+    }
+    if (k == p.nrd - 1) {
+      off += (int64_t)q * p.str[k];
+      break;
+    }
+    const uint32_t nq = p.div[k].div(q);
+    off += (int64_t)(q - nq * p.cnt[k]) * p.str[k];
+    q = nq;
+  }
+  return off;
+}
+
+// Move work unit `u` (wave-uniform when P == 64).  DIR 0 = pack
+// (strided -> packed), 1 = unpack (packed -> strided).
+template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
+__device__ __forceinline__ void copy_unit(const KParams &p, uint32_t u,
+                                          uint32_t lane) {
+  typedef typename WordT<W>::T T;
+  constexpr int A = word_align(W);
+  const uint32_t row = p.nseg == 1 ? u : p.segdiv.div(u);
+  const uint32_t seg = u - row * p.nseg;
+  const int64_t soff = row_offset(p, row);
+  const int64_t poff = (int64_t)row * p.L;
+  const char *s = p.src + (DIR == 0 ? soff : poff);
+  char *d = p.dst + (DIR == 0 ? poff : soff);
+  const uint32_t w0 = seg * (uint32_t)(P * U) + lane;
+  if constexpr (ALIGNED) {
+    T v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t w = w0 + k * P;
+      if (w < p.words) {
+        v[k] = load_word<W>(s + (size_t)w * W);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t w = w0 + k * P;
+      if (w < p.words) {
+        store_word<W, NT>(d + (size_t)w * W, v[k]);
+      }
+    }
+  } else {
+    const uint32_t mis = (uint32_t)((uintptr_t)d & (A - 1));
+    uint32_t head = mis ? (uint32_t)(A - mis) : 0u;
+    if ((int64_t)head > p.L) {
+      head = (uint32_t)p.L;
+    }
+    const uint32_t body = (uint32_t)((p.L - head) / W);
+    const int64_t t0 = (int64_t)head + (int64_t)body * W;
+    const uint32_t tail = (uint32_t)(p.L - t0);
+    const char *sb = s + head;
+    char *db = d + head;
+    T v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t w = w0 + k * P;
+      if (w < body) {
+        v[k] = load_word<W>(sb + (size_t)w * W);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t w = w0 + k * P;
+      if (w < body) {
+        store_word<W, NT>(db + (size_t)w * W, v[k]);
+      }
+    }
+    if (seg == 0) {
+      for (uint32_t b = lane; b < head; b += P) {
+        d[b] = s[b];
+      }
+      for (uint32_t b = lane; b < tail; b += P) {
+        d[t0 + b] = s[t0 + b];
+      }
+    }
+  }
+}
+
+constexpr int kBlock = 256;
+
+// The variant list, as X(index, P, W, U, NT).
+#define TEMPI_VARIANTS(X)                                                      \
+  X(0, 64, 16, 1, false) /* long rows, 1 KiB per wavefront unit        */     \
+  X(1, 64, 16, 4, false) /* long rows, 4 KiB per unit                  */     \
+  X(2, 64, 16, 4, true)  /* as 1 with nontemporal stores               */     \
+  X(3, 32, 16, 2, false) /* rows around 512 B - 1 KiB                  */     \
+  X(4, 16, 16, 2, false) /* rows around 256 - 512 B                    */     \
+  X(5, 8, 16, 2, false)  /* rows around 128 - 256 B                    */     \
+  X(6, 4, 16, 2, false)  /* rows around 64 - 128 B                     */     \
+  X(7, 1, 16, 4, false)  /* lane per row, 16-B words                   */     \
+  X(8, 1, 8, 4, false)   /* lane per row, 8-B words                    */     \
+  X(9, 1, 4, 4, false)   /* lane per row, 4-B words                    */     \
+  X(10, 16, 4, 4, false) /* 4-B words across 16 lanes                  */     \
+  X(11, 1, 1, 16, false) /* lane per row, bytes                        */     \
+  X(12, 1, 12, 2, false) /* lane per row, 12-B words (3-float rows)    */
+
+// Fill the kernel parameters of one copy with `variant`; `units` may not fit
+// 32 bits, which is reported as TEMPI_ERR_UNSUPPORTED.
+int fill_kparams(const Geom &g, const char *strided, char *packed, int dir,
+                 int variant, KParams *p, bool *aligned);
+
+} // namespace tempi

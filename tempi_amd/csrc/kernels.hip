@@ -20,7 +20,7 @@
 // the word width honours the runtime pointers and strides, not only the
 // block length and start (:129-157), and the object loop is part of the
 // parallel row space instead of a serial loop (:372).
-#include "internal.hpp"
+#include "copy_core.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -28,20 +28,7 @@
 
 namespace tempi {
 
-// n / d for 32-bit n by multiply-high and shift (Granlund and Montgomery,
-// "Division by Invariant Integers using Multiplication", PLDI 1994, fig. 4.1):
-// with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
-// n / d = (mulhi(n, m) + n) >> l, the sum taken in 64 bits.
-struct FastDiv {
-  uint32_t mul = 1;
-  uint32_t shift = 0;
-  __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const {
-    const uint32_t t = (uint32_t)(((uint64_t)n * mul) >> 32);
-    return (uint32_t)(((uint64_t)t + n) >> shift);
-  }
-};
-
-static FastDiv make_fastdiv(uint32_t d) {
+FastDiv make_fastdiv(uint32_t d) {
   FastDiv f;
   uint32_t l = 0;
   while (l < 32 && (1ull << l) < d) {
@@ -54,75 +41,8 @@ static FastDiv make_fastdiv(uint32_t d) {
   return f;
 }
 
-struct KParams {
-  const char *src;
-  char *dst;
-  int64_t L;      // bytes per row
-  uint32_t units; // rows * nseg
-  uint32_t nseg;  // segments per row
-  FastDiv segdiv;
-  uint32_t words; // L / W
-  int32_t nrd;    // row dimensions
-  uint32_t cnt[kMaxRowDims];
-  FastDiv div[kMaxRowDims];
-  int64_t str[kMaxRowDims];
-};
-
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-
-template <int W> struct WordT;
-template <> struct WordT<16> { typedef u32x4 T; };
-template <> struct WordT<8> { typedef u32x2 T; };
-template <> struct WordT<4> { typedef uint32_t T; };
-template <> struct WordT<2> { typedef uint16_t T; };
-template <> struct WordT<1> { typedef uint8_t T; };
-
-// load W bytes from any byte address
-template <int W>
-__device__ __forceinline__ typename WordT<W>::T load_word(const char *p) {
-  typedef typename WordT<W>::T T;
-  typedef T Tu __attribute__((aligned(1)));
-  return *reinterpret_cast<const Tu *>(p);
-}
-
-// store W bytes to a W-aligned address
-template <int W, bool NT>
-__device__ __forceinline__ void store_word(char *p, typename WordT<W>::T v) {
-  typedef typename WordT<W>::T T;
-  if constexpr (NT) {
-    __builtin_nontemporal_store(v, reinterpret_cast<T *>(p));
-  } else {
-    *reinterpret_cast<T *>(p) = v;
-  }
-}
-
-// byte offset of row `row` on the strided side
-__device__ __forceinline__ int64_t row_offset(const KParams &p, uint32_t row) {
-  int64_t off = 0;
-  uint32_t q = row;
-#pragma unroll
-  for (int k = 0; k < kMaxRowDims; ++k) {
-    if (k >= p.nrd) {
-      break;
-    }
-    if (k == p.nrd - 1) {
-      off += (int64_t)q * p.str[k];
-      break;
-    }
-    const uint32_t nq = p.div[k].div(q);
-    off += (int64_t)(q - nq * p.cnt[k]) * p.str[k];
-    q = nq;
-  }
-  return off;
-}
-
-constexpr int kBlock = 256;
-
-// DIR 0 = pack (strided -> packed), 1 = unpack (packed -> strided)
 template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
 __global__ void __launch_bounds__(kBlock) k_strided_copy(KParams p) {
-  typedef typename WordT<W>::T T;
   constexpr uint32_t GROUPS = kBlock / P;
   const uint32_t lane = threadIdx.x % P;
   const uint32_t step = gridDim.x * GROUPS;
@@ -132,64 +52,7 @@ __global__ void __launch_bounds__(kBlock) k_strided_copy(KParams p) {
     if constexpr (P == 64) {
       uu = __builtin_amdgcn_readfirstlane(u); // wave-uniform: scalar math
     }
-    const uint32_t row = p.nseg == 1 ? uu : p.segdiv.div(uu);
-    const uint32_t seg = uu - row * p.nseg;
-    const int64_t soff = row_offset(p, row);
-    const int64_t poff = (int64_t)row * p.L;
-    const char *s = p.src + (DIR == 0 ? soff : poff);
-    char *d = p.dst + (DIR == 0 ? poff : soff);
-    const uint32_t w0 = seg * (uint32_t)(P * U) + lane;
-    if constexpr (ALIGNED) {
-      T v[U];
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const uint32_t w = w0 + k * P;
-        if (w < p.words) {
-          v[k] = *reinterpret_cast<const T *>(s + (size_t)w * W);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const uint32_t w = w0 + k * P;
-        if (w < p.words) {
-          store_word<W, NT>(d + (size_t)w * W, v[k]);
-        }
-      }
-    } else {
-      const uint32_t mis = (uint32_t)((uintptr_t)d & (W - 1));
-      uint32_t head = mis ? (uint32_t)(W - mis) : 0u;
-      if ((int64_t)head > p.L) {
-        head = (uint32_t)p.L;
-      }
-      const uint32_t body = (uint32_t)((p.L - head) / W);
-      const int64_t t0 = (int64_t)head + (int64_t)body * W;
-      const uint32_t tail = (uint32_t)(p.L - t0);
-      const char *sb = s + head;
-      char *db = d + head;
-      T v[U];
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const uint32_t w = w0 + k * P;
-        if (w < body) {
-          v[k] = load_word<W>(sb + (size_t)w * W);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const uint32_t w = w0 + k * P;
-        if (w < body) {
-          store_word<W, NT>(db + (size_t)w * W, v[k]);
-        }
-      }
-      if (seg == 0) {
-        for (uint32_t b = lane; b < head; b += P) {
-          d[b] = s[b];
-        }
-        for (uint32_t b = lane; b < tail; b += P) {
-          d[t0 + b] = s[t0 + b];
-        }
-      }
-    }
+    copy_unit<P, W, U, ALIGNED, DIR, NT>(p, uu, lane);
   }
 }
 
@@ -197,39 +60,21 @@ __global__ void __launch_bounds__(kBlock) k_strided_copy(KParams p) {
 // variant table
 // ---------------------------------------------------------------------------
 
-// lanes per row, word bytes, words per lane per unit, nontemporal stores
-const VariantDesc kVariants[] = {
-    {64, 16, 1, 0}, // 0: long rows, 1 KiB per wavefront unit
-    {64, 16, 4, 0}, // 1: long rows, 4 KiB per unit (more loads in flight)
-    {64, 16, 4, 1}, // 2: as 1 with nontemporal stores
-    {32, 16, 2, 0}, // 3: rows around 512 B - 1 KiB
-    {16, 16, 2, 0}, // 4: rows around 256 - 512 B
-    {8, 16, 2, 0},  // 5: rows around 128 - 256 B
-    {4, 16, 2, 0},  // 6: rows around 64 - 128 B
-    {1, 16, 4, 0},  // 7: lane per row (short, 16-B words)
-    {1, 8, 4, 0},   // 8: lane per row, 8-B words
-    {1, 4, 4, 0},   // 9: lane per row, 4-B words (12-B halo rows)
-    {16, 4, 4, 0},  // 10: 4-B words across 16 lanes
-    {1, 1, 16, 0},  // 11: lane per row, bytes (odd short rows)
-};
+#define TEMPI_VDESC(I, P, W, U, NT) {P, W, U, NT ? 1 : 0},
+const VariantDesc kVariants[] = {TEMPI_VARIANTS(TEMPI_VDESC)};
+#undef TEMPI_VDESC
 const int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 typedef void (*KernelFn)(KParams);
 
-#define TEMPI_KROW(P, W, U, NT)                                                \
+#define TEMPI_KROW(I, P, W, U, NT)                                                \
   {{k_strided_copy<P, W, U, false, 0, NT>,                                     \
     k_strided_copy<P, W, U, false, 1, NT>},                                    \
-   {k_strided_copy<P, W, U, true, 0, NT>, k_strided_copy<P, W, U, true, 1, NT>}}
+   {k_strided_copy<P, W, U, true, 0, NT>,                                      \
+    k_strided_copy<P, W, U, true, 1, NT>}},
 
 // [variant][aligned][dir]
-static const KernelFn kKernels[][2][2] = {
-    TEMPI_KROW(64, 16, 1, false), TEMPI_KROW(64, 16, 4, false),
-    TEMPI_KROW(64, 16, 4, true),  TEMPI_KROW(32, 16, 2, false),
-    TEMPI_KROW(16, 16, 2, false), TEMPI_KROW(8, 16, 2, false),
-    TEMPI_KROW(4, 16, 2, false),  TEMPI_KROW(1, 16, 4, false),
-    TEMPI_KROW(1, 8, 4, false),   TEMPI_KROW(1, 4, 4, false),
-    TEMPI_KROW(16, 4, 4, false),  TEMPI_KROW(1, 1, 16, false),
-};
+static const KernelFn kKernels[][2][2] = {TEMPI_VARIANTS(TEMPI_KROW)};
 #undef TEMPI_KROW
 static_assert(sizeof(kKernels) / sizeof(kKernels[0]) ==
                   sizeof(kVariants) / sizeof(kVariants[0]),
@@ -285,14 +130,15 @@ int make_geom(const tempi_strided_block &sb, int64_t count, Geom *g) {
 
 bool geom_aligned(const Geom &g, const char *strided, const char *packed,
                   int W) {
+  const int A = word_align(W);
   if (g.L % W) {
     return false;
   }
-  if ((uintptr_t)packed % W || (uintptr_t)(strided + g.start) % W) {
+  if ((uintptr_t)packed % A || (uintptr_t)(strided + g.start) % A) {
     return false;
   }
   for (int k = 0; k < g.nrd; ++k) {
-    if (g.str[k] % W) {
+    if (g.str[k] % A) {
       return false;
     }
   }
@@ -337,16 +183,16 @@ int heuristic_variant(const Geom &g, const char *strided, const char *packed,
     return 8;
   }
   if (Wa == 4) {
-    return 9;
+    // rows of 3 or 6 dwords move as 12-B words
+    return (L % 12 == 0) ? 12 : 9;
   }
   return L >= 16 ? 7 : 11;
 }
 
-static int launch_one(const Geom &g, const char *strided, char *packed,
-                      int dir, int variant, hipStream_t stream,
-                      int max_blocks) {
+int fill_kparams(const Geom &g, const char *strided, char *packed, int dir,
+                 int variant, KParams *pp, bool *aligned) {
   const VariantDesc &v = kVariants[variant];
-  const bool aligned = geom_aligned(g, strided, packed, v.W);
+  *aligned = geom_aligned(g, strided, packed, v.W);
   const int64_t words = g.L / v.W;
   const int64_t per_unit = (int64_t)v.P * v.U;
   const int64_t nseg = std::max<int64_t>(1, (words + per_unit - 1) / per_unit);
@@ -356,7 +202,8 @@ static int launch_one(const Geom &g, const char *strided, char *packed,
                      "copy of %lld rows of %lld B exceeds the 2^32 unit space",
                      (long long)g.rows, (long long)g.L);
   }
-  KParams p{};
+  KParams &p = *pp;
+  p = KParams{};
   const char *sbase = strided + g.start;
   if (dir == TEMPI_DIR_PACK) {
     p.src = sbase;
@@ -379,8 +226,20 @@ static int launch_one(const Geom &g, const char *strided, char *packed,
     p.div[k] = make_fastdiv((uint32_t)g.cnt[k]);
     p.str[k] = g.str[k];
   }
-  const int64_t groups = kBlock / v.P;
-  int64_t blocks = (units + groups - 1) / groups;
+  return TEMPI_SUCCESS;
+}
+
+static int launch_one(const Geom &g, const char *strided, char *packed,
+                      int dir, int variant, hipStream_t stream,
+                      int max_blocks) {
+  KParams p;
+  bool aligned = false;
+  int err = fill_kparams(g, strided, packed, dir, variant, &p, &aligned);
+  if (err) {
+    return err;
+  }
+  const int64_t groups = kBlock / kVariants[variant].P;
+  int64_t blocks = ((int64_t)p.units + groups - 1) / groups;
   const int64_t cap =
       max_blocks > 0 ? max_blocks : (int64_t)device_cu_count() * 16;
   blocks = std::min<int64_t>(blocks, cap);
