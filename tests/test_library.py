@@ -48,7 +48,7 @@ def test_library_loads_and_reports():
     assert ta.num_variants() >= 12
     for v in range(ta.num_variants()):
         info = ta.variant_info(v)
-        assert info["word_bytes"] in (1, 2, 4, 8, 16)
+        assert info["word_bytes"] in (1, 2, 4, 8, 12, 16)
         assert 64 % info["lanes_per_row"] == 0
     with pytest.raises(ta.TempiError):
         ta.variant_info(ta.num_variants())
