@@ -114,7 +114,11 @@ def test_heuristic_selection_is_host_logic():
     # strided / packed addresses only enter through their alignment
     sb = ta.StridedBlock.make(3226116, 555967328, [12, 512, 512], [1, 2072, 1073296])
     v = ta.select_variant(sb, 1, 1 << 20, 1 << 20)
-    assert ta.variant_info(v)["word_bytes"] == 4
+    assert ta.variant_info(v)["word_bytes"] == 12  # 3-float rows, 4-B aligned
+    assert ta.variant_info(v)["lanes_per_row"] == 1
+    sb = ta.StridedBlock.make(3226116 + 1, 555967328, [12, 512, 512], [1, 2072, 1073296])
+    v = ta.select_variant(sb, 1, 1 << 20, 1 << 20)
+    assert ta.variant_info(v)["word_bytes"] == 1  # byte-aligned start
     sb = ta.StridedBlock.make(0, 524032, [256, 1024], [1, 512])
     v = ta.select_variant(sb, 1, 1 << 20, 1 << 20)
     assert ta.variant_info(v)["word_bytes"] == 16
