@@ -4,12 +4,12 @@
 // (src/internal/packer_2d.cu:101-108).  A batch instead resolves every
 // request once, on the host, into the same per-copy parameters the single
 // kernels take, uploads them as a device-resident table, and splits each
-// request into tiles of about kTileBytes.  A persistent grid (a few
+// request into tiles of one workgroup pass (256 threads, four loads in
+// flight per lane: 16 KiB for 16-byte words).  A persistent grid (a few
 // workgroups per CU) drains the tile queue: each workgroup takes the next
 // tile index with one atomic add (fetched one tile ahead so the atomic's
-// latency overlaps the copy), finds the owning request by binary search over
-// the tiles' prefix sums, and runs that request's kernel variant on the
-// tile's units.  Heterogeneous shapes -- 12-B halo rows next to 2-KiB face
+// latency overlaps the copy), reads the owning request from a tile-to-request
+// table, and runs that request's kernel variant on the tile's units.  Heterogeneous shapes -- 12-B halo rows next to 2-KiB face
 // rows -- share one launch.
 //
 // The queue counter is reset by the last workgroup to leave, so the launch
@@ -29,12 +29,10 @@ namespace tempi {
 
 namespace {
 
-constexpr int64_t kTileBytes = 32 << 10;
-
 struct BatchItem {
   KParams p;
   int32_t kind;            // 4 * variant + 2 * aligned + direction
-  uint32_t units_per_tile; // a multiple of the variant's groups per block
+  uint32_t units_per_tile; // units of one block-pass: groups per block * K
 };
 
 struct BatchCtl {
@@ -42,35 +40,29 @@ struct BatchCtl {
   uint32_t done; // workgroups that have left
 };
 
+// one block-pass over units [u0, u0 + GROUPS*K) of a request
 template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
-__device__ __forceinline__ void copy_range(const KParams &p, uint32_t u0,
-                                           uint32_t u1) {
+__device__ __forceinline__ void copy_tile(const KParams &p, uint32_t u0) {
   constexpr uint32_t GROUPS = kBlock / P;
-  const uint32_t lane = threadIdx.x % P;
-  for (uint32_t u = u0 + threadIdx.x / P; u < u1; u += GROUPS) {
-    uint32_t uu = u;
-    if constexpr (P == 64) {
-      uu = __builtin_amdgcn_readfirstlane(u);
-    }
-    copy_unit<P, W, U, ALIGNED, DIR, NT>(p, uu, lane);
-  }
+  copy_units<P, W, U, ALIGNED, DIR, NT>(p, u0 + threadIdx.x / P, GROUPS,
+                                        p.units, threadIdx.x % P);
 }
 
-__device__ __forceinline__ void run_range(int kind, const KParams &p,
-                                          uint32_t u0, uint32_t u1) {
+__device__ __forceinline__ void run_tile(int kind, const KParams &p,
+                                         uint32_t u0) {
   switch (kind) {
 #define TEMPI_BCASE(I, P, W, U, NT)                                            \
   case 4 * I + 0:                                                              \
-    copy_range<P, W, U, false, 0, NT>(p, u0, u1);                              \
+    copy_tile<P, W, U, false, 0, NT>(p, u0);                              \
     break;                                                                     \
   case 4 * I + 1:                                                              \
-    copy_range<P, W, U, false, 1, NT>(p, u0, u1);                              \
+    copy_tile<P, W, U, false, 1, NT>(p, u0);                              \
     break;                                                                     \
   case 4 * I + 2:                                                              \
-    copy_range<P, W, U, true, 0, NT>(p, u0, u1);                               \
+    copy_tile<P, W, U, true, 0, NT>(p, u0);                               \
     break;                                                                     \
   case 4 * I + 3:                                                              \
-    copy_range<P, W, U, true, 1, NT>(p, u0, u1);                               \
+    copy_tile<P, W, U, true, 1, NT>(p, u0);                               \
     break;
     TEMPI_VARIANTS(TEMPI_BCASE)
 #undef TEMPI_BCASE
@@ -80,8 +72,9 @@ __device__ __forceinline__ void run_range(int kind, const KParams &p,
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_batch(const BatchItem *__restrict__ items,
-            const uint32_t *__restrict__ tile_begin, int n, uint32_t ntiles,
+    __attribute__((amdgpu_waves_per_eu(5))) k_batch(const BatchItem *__restrict__ items,
+            const uint32_t *__restrict__ tile_begin,
+            const uint32_t *__restrict__ tile_item, uint32_t ntiles,
             BatchCtl *ctl) {
   __shared__ uint32_t s_next;
   if (threadIdx.x == 0) {
@@ -94,23 +87,13 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x == 0) {
       s_next = atomicAdd(&ctl->next, 1u); // one tile ahead
     }
-    // the request owning tile t: last r with tile_begin[r] <= t
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (tile_begin[mid] <= t) {
-        lo = mid;
-      } else {
-        hi = mid - 1;
-      }
-    }
-    lo = __builtin_amdgcn_readfirstlane(lo);
-    const KParams p = items[lo].p;
-    const int kind = items[lo].kind;
-    const uint32_t upt = items[lo].units_per_tile;
-    const uint32_t u0 = (t - tile_begin[lo]) * upt;
-    const uint32_t u1 = min(u0 + upt, p.units);
-    run_range(kind, p, u0, u1);
+    // the request owning tile t and the tile's first unit
+    const uint32_t r = __builtin_amdgcn_readfirstlane(tile_item[t]);
+    // read the request's parameters in place (uniform address: scalar
+    // loads); a private copy of the struct would live in scratch memory
+    const BatchItem &it = items[r];
+    const uint32_t u0 = (t - tile_begin[r]) * it.units_per_tile;
+    run_tile(it.kind, it.p, u0);
     __syncthreads();
     t = __builtin_amdgcn_readfirstlane(s_next);
     __syncthreads();
@@ -137,9 +120,10 @@ struct tempi_batch {
   uint32_t ntiles = 0;
   int grid = 0;
   int n = 0; // items with work
-  char *dev = nullptr; // items, tile_begin, ctl in one allocation
+  char *dev = nullptr; // items, tile tables, ctl in one allocation
   BatchItem *d_items = nullptr;
-  uint32_t *d_tile_begin = nullptr;
+  uint32_t *d_tile_begin = nullptr; // first tile of each item
+  uint32_t *d_tile_item = nullptr;  // item of each tile
   BatchCtl *d_ctl = nullptr;
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -152,7 +136,7 @@ static int batch_launch_on(tempi_batch *b, hipStream_t s) {
     return TEMPI_SUCCESS;
   }
   hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0, s, b->d_items,
-                     b->d_tile_begin, b->n, b->ntiles, b->d_ctl);
+                     b->d_tile_begin, b->d_tile_item, b->ntiles, b->d_ctl);
   TEMPI_HIP_CHECK(hipGetLastError());
   return TEMPI_SUCCESS;
 }
@@ -171,6 +155,7 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   b->requests = n;
   std::vector<BatchItem> items;
   std::vector<uint32_t> tile_begin;
+  std::vector<uint32_t> tile_item;
   uint64_t tiles = 0;
   for (int64_t i = 0; i < n; ++i) {
     const tempi_batch_request &r = reqs[i];
@@ -214,20 +199,16 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
     }
     it.kind = 4 * variant + 2 * (aligned ? 1 : 0) + dir;
     const VariantDesc &v = kVariants[variant];
-    const int64_t groups = kBlock / v.P;
-    const int64_t unit_bytes =
-        std::max<int64_t>(1, std::min<int64_t>(g.L, (int64_t)v.P * v.U * v.W));
-    int64_t upt = std::max<int64_t>(groups, kTileBytes / unit_bytes);
-    upt = (upt + groups - 1) / groups * groups;
-    it.units_per_tile = (uint32_t)std::min<int64_t>(upt, 1ll << 30);
+    it.units_per_tile = (uint32_t)(kBlock / v.P) * units_in_flight(v.U);
     const uint64_t t = ((uint64_t)it.p.units + it.units_per_tile - 1) /
                        it.units_per_tile;
     tile_begin.push_back((uint32_t)tiles);
     tiles += t;
-    if (tiles >= (1ull << 32)) {
+    if (tiles >= (1ull << 31)) {
       delete b;
-      return set_error(TEMPI_ERR_UNSUPPORTED, "batch exceeds 2^32 tiles");
+      return set_error(TEMPI_ERR_UNSUPPORTED, "batch exceeds 2^31 tiles");
     }
+    tile_item.insert(tile_item.end(), t, (uint32_t)items.size());
     items.push_back(it);
     b->bytes += g.L * g.rows;
   }
@@ -237,7 +218,8 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
       1, std::min<int64_t>((int64_t)tiles, (int64_t)device_cu_count() * 8));
   const size_t items_bytes = items.size() * sizeof(BatchItem);
   const size_t tb_off = (items_bytes + 255) / 256 * 256;
-  const size_t ctl_off = tb_off + (tile_begin.size() * 4 + 255) / 256 * 256;
+  const size_t ti_off = tb_off + (tile_begin.size() * 4 + 255) / 256 * 256;
+  const size_t ctl_off = ti_off + (tile_item.size() * 4 + 255) / 256 * 256;
   const size_t total = ctl_off + 256;
   if (hipSuccess != hipMalloc(&b->dev, total)) {
     (void)hipGetLastError();
@@ -246,12 +228,14 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   }
   b->d_items = reinterpret_cast<BatchItem *>(b->dev);
   b->d_tile_begin = reinterpret_cast<uint32_t *>(b->dev + tb_off);
+  b->d_tile_item = reinterpret_cast<uint32_t *>(b->dev + ti_off);
   b->d_ctl = reinterpret_cast<BatchCtl *>(b->dev + ctl_off);
   hipStream_t s = library_stream();
   std::vector<char> host(total, 0);
   if (!items.empty()) {
     memcpy(host.data(), items.data(), items_bytes);
     memcpy(host.data() + tb_off, tile_begin.data(), tile_begin.size() * 4);
+    memcpy(host.data() + ti_off, tile_item.data(), tile_item.size() * 4);
   }
   // one upload (the counter words are zero in `host`), outside any capture
   if (hipSuccess != hipMemcpyAsync(b->dev, host.data(), total,
@@ -291,8 +275,8 @@ TEMPI_EXPORT int tempi_batch_graph_launch(tempi_batch *b, void *stream) {
     TEMPI_HIP_CHECK(hipStreamBeginCapture(b->capture_stream,
                                           hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0,
-                       b->capture_stream, b->d_items, b->d_tile_begin, b->n,
-                       b->ntiles, b->d_ctl);
+                       b->capture_stream, b->d_items, b->d_tile_begin,
+                       b->d_tile_item, b->ntiles, b->d_ctl);
     TEMPI_HIP_CHECK(hipStreamEndCapture(b->capture_stream, &b->graph));
     TEMPI_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }

@@ -3,7 +3,8 @@
 //
 // A copy is `units` work units; unit u is segment (u % nseg) of row
 // (u / nseg).  A group of P lanes moves one unit as U words of W bytes per
-// lane, the P lanes touching consecutive words.  Stores are always aligned
+// lane, the P lanes touching consecutive words; each thread issues the loads
+// of K = units_in_flight(U) units before storing any of them.  Stores are always aligned
 // to the word's natural unit A (W for powers of two, 4 for 12-byte words);
 // loads may sit at any byte address (gfx9 global loads accept that).
 #pragma once
@@ -100,12 +101,22 @@ __device__ __forceinline__ int64_t row_offset(const KParams &p, uint32_t row) {
   return off;
 }
 
-// Move work unit `u` (wave-uniform when P == 64).  DIR 0 = pack
-// (strided -> packed), 1 = unpack (packed -> strided).
-template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
-__device__ __forceinline__ void copy_unit(const KParams &p, uint32_t u,
-                                          uint32_t lane) {
-  typedef typename WordT<W>::T T;
+// Loads of one work unit held in registers between the load and store
+// phases, so that a thread can have several units' loads in flight before
+// its first store.
+template <int W, int U> struct UnitIO {
+  typename WordT<W>::T v[U];
+  char *d;    // first body byte on the destination side
+  uint32_t w; // first word index of this lane
+  uint32_t n; // body words of the row
+};
+
+// Load phase of work unit `u` (wave-uniform when P == 64).  DIR 0 = pack
+// (strided -> packed), 1 = unpack (packed -> strided).  Head and tail bytes
+// of a row whose destination is not word-aligned are copied here directly.
+template <int P, int W, int U, bool ALIGNED, int DIR>
+__device__ __forceinline__ void load_unit(const KParams &p, uint32_t u,
+                                          uint32_t lane, UnitIO<W, U> &io) {
   constexpr int A = word_align(W);
   const uint32_t row = p.nseg == 1 ? u : p.segdiv.div(u);
   const uint32_t seg = u - row * p.nseg;
@@ -113,56 +124,79 @@ __device__ __forceinline__ void copy_unit(const KParams &p, uint32_t u,
   const int64_t poff = (int64_t)row * p.L;
   const char *s = p.src + (DIR == 0 ? soff : poff);
   char *d = p.dst + (DIR == 0 ? poff : soff);
-  const uint32_t w0 = seg * (uint32_t)(P * U) + lane;
+  io.w = seg * (uint32_t)(P * U) + lane;
   if constexpr (ALIGNED) {
-    T v[U];
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const uint32_t w = w0 + k * P;
-      if (w < p.words) {
-        v[k] = load_word<W>(s + (size_t)w * W);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const uint32_t w = w0 + k * P;
-      if (w < p.words) {
-        store_word<W, NT>(d + (size_t)w * W, v[k]);
-      }
-    }
+    io.n = p.words;
   } else {
     const uint32_t mis = (uint32_t)((uintptr_t)d & (A - 1));
     uint32_t head = mis ? (uint32_t)(A - mis) : 0u;
     if ((int64_t)head > p.L) {
       head = (uint32_t)p.L;
     }
-    const uint32_t body = (uint32_t)((p.L - head) / W);
-    const int64_t t0 = (int64_t)head + (int64_t)body * W;
-    const uint32_t tail = (uint32_t)(p.L - t0);
-    const char *sb = s + head;
-    char *db = d + head;
-    T v[U];
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const uint32_t w = w0 + k * P;
-      if (w < body) {
-        v[k] = load_word<W>(sb + (size_t)w * W);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const uint32_t w = w0 + k * P;
-      if (w < body) {
-        store_word<W, NT>(db + (size_t)w * W, v[k]);
-      }
-    }
+    io.n = (uint32_t)((p.L - head) / W);
     if (seg == 0) {
+      const int64_t t0 = (int64_t)head + (int64_t)io.n * W;
+      const uint32_t tail = (uint32_t)(p.L - t0);
       for (uint32_t b = lane; b < head; b += P) {
         d[b] = s[b];
       }
       for (uint32_t b = lane; b < tail; b += P) {
         d[t0 + b] = s[t0 + b];
       }
+    }
+    s += head;
+    d += head;
+  }
+  io.d = d;
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const uint32_t w = io.w + k * P;
+    if (w < io.n) {
+      io.v[k] = load_word<W>(s + (size_t)w * W);
+    }
+  }
+}
+
+template <int P, int W, int U, bool NT>
+__device__ __forceinline__ void store_unit(const UnitIO<W, U> &io) {
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const uint32_t w = io.w + k * P;
+    if (w < io.n) {
+      store_word<W, NT>(io.d + (size_t)w * W, io.v[k]);
+    }
+  }
+}
+
+// units a thread keeps in flight: enough for four loads per lane
+__host__ __device__ constexpr int units_in_flight(int U) {
+  return U >= 4 ? 1 : 4 / U;
+}
+
+// Move units first, first + stride, ... (K of them, those below `end`):
+// all loads, then all stores.
+template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
+__device__ __forceinline__ void copy_units(const KParams &p, uint32_t first,
+                                           uint32_t stride, uint32_t end,
+                                           uint32_t lane) {
+  constexpr int K = units_in_flight(U);
+  UnitIO<W, U> io[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    uint32_t u = first + k * stride;
+    if constexpr (P == 64 && K == 1) {
+      // wave-uniform: the row arithmetic runs on the scalar unit (with
+      // several units in flight their state would overflow the SGPRs)
+      u = __builtin_amdgcn_readfirstlane(u);
+    }
+    if (u < end) {
+      load_unit<P, W, U, ALIGNED, DIR>(p, u, lane, io[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (first + k * stride < end) {
+      store_unit<P, W, U, NT>(io[k]);
     }
   }
 }

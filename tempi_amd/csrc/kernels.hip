@@ -44,15 +44,12 @@ FastDiv make_fastdiv(uint32_t d) {
 template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
 __global__ void __launch_bounds__(kBlock) k_strided_copy(KParams p) {
   constexpr uint32_t GROUPS = kBlock / P;
+  constexpr uint32_t K = units_in_flight(U);
   const uint32_t lane = threadIdx.x % P;
   const uint32_t step = gridDim.x * GROUPS;
   for (uint32_t u = blockIdx.x * GROUPS + threadIdx.x / P; u < p.units;
-       u += step) {
-    uint32_t uu = u;
-    if constexpr (P == 64) {
-      uu = __builtin_amdgcn_readfirstlane(u); // wave-uniform: scalar math
-    }
-    copy_unit<P, W, U, ALIGNED, DIR, NT>(p, uu, lane);
+       u += step * K) {
+    copy_units<P, W, U, ALIGNED, DIR, NT>(p, u, step, p.units, lane);
   }
 }
 
@@ -238,7 +235,8 @@ static int launch_one(const Geom &g, const char *strided, char *packed,
   if (err) {
     return err;
   }
-  const int64_t groups = kBlock / kVariants[variant].P;
+  const int64_t groups = (int64_t)(kBlock / kVariants[variant].P) *
+                         units_in_flight(kVariants[variant].U);
   int64_t blocks = ((int64_t)p.units + groups - 1) / groups;
   const int64_t cap =
       max_blocks > 0 ? max_blocks : (int64_t)device_cu_count() * 16;
