@@ -176,8 +176,8 @@ int tempi_launch_latency_us(int reps, float *host_us, float *device_us);
    device-resident descriptor table; one launch runs a persistent kernel that
    drains all of them.  tempi_batch_graph_launch replays the same launch from
    a captured hipGraph.  Requests in one batch must be independent (no
-   request writes bytes another reads or writes); launches of one batch must
-   be ordered (same stream).  Nothing is synchronized: wait on the stream. */
+   request writes bytes another reads or writes).  Nothing is synchronized:
+   wait on the stream. */
 typedef struct tempi_batch_request {
   tempi_strided_block sb; /* layout of one object                          */
   const void *strided;    /* strided buffer origin (pack: in, unpack: out)  */
@@ -196,6 +196,45 @@ int tempi_batch_info(const tempi_batch *batch, int64_t *requests,
 int tempi_batch_variant(const tempi_batch *batch, int64_t request,
                         int *variant);
 int tempi_batch_destroy(tempi_batch *batch);
+
+/* Learned variant selector: replaces the reference's measured tables and
+   interpolation (src/internal/measure_system.cpp:181-299) and its fixed
+   kernel rule (include/pack_kernels.cuh:129-166).  An MLP
+   16 -> 64 -> 64 -> tempi_mlp_num_outputs() (ReLU, fp32) predicts the
+   relative time (log2, centred) of each kernel variant; the selector takes
+   the argmin.  Parameters are laid out [W1 16x64, b1, W2 64x64, b2, W3 64xV,
+   b3] (row = input); `norm` is the feature mean[16] then 1/std[16].
+   tempi_mlp_features: the 16 features of a copy (log2 row bytes, rows,
+     total bytes; row dimensions; log2 of the first two dimensions' counts and
+     strides and of the remaining count; log2 alignment (capped at 16 B) of
+     the strided start, packed pointer, strides and row length; direction;
+     12-byte rows; log2 stride/row ratio).
+   tempi_mlp_grad: loss and gradient of one minibatch (n <= 32) of
+     normalized features X[n][16], targets T[n][V] and optional mask M, by
+     the GPU kernel training uses.
+   tempi_mlp_train: Adam over minibatches of 32 for `epochs` epochs, all on
+     the GPU; `params` holds the initial values and receives the result.
+   tempi_mlp_predict: the inference kernel on raw features.
+   tempi_mlp_save / tempi_mlp_load: versioned text file.  The library loads
+     $TEMPI_MODEL, else selector_mlp.txt next to itself, on first use; a
+     missing or malformed file leaves the closed-form rule in place.
+   With a model loaded, MPI_Type_commit evaluates it for the new type. */
+int tempi_mlp_num_features(void);
+int tempi_mlp_num_outputs(void);
+int tempi_mlp_num_params(void);
+int tempi_mlp_features(const tempi_strided_block *sb, int64_t incount,
+                       const void *strided, const void *packed, int direction,
+                       float *features);
+int tempi_mlp_grad(const float *params, const float *X, const float *T,
+                   const float *M, int n, float *grad, float *loss);
+int tempi_mlp_train(const float *X, const float *T, const float *M, int64_t n,
+                    int epochs, float lr, uint64_t seed, float *params,
+                    float *epoch_loss);
+int tempi_mlp_predict(const float *params, const float *norm, const float *X,
+                      int64_t n, float *pred, int *best);
+int tempi_mlp_save(const char *path, const float *params, const float *norm);
+int tempi_mlp_load(const char *path);
+int tempi_mlp_loaded(void);
 
 /* The library's non-blocking stream (reference: kernStream,
    src/internal/streams.cpp:25), created on first use. */

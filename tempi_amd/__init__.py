@@ -182,6 +182,16 @@ def _prototypes(L):
         "tempi_batch_info": (_I, [_P, i64_p, i64_p, i64_p]),
         "tempi_batch_variant": (_I, [_P, _I64, ctypes.POINTER(_I)]),
         "tempi_batch_destroy": (_I, [_P]),
+        "tempi_mlp_num_features": (_I, []),
+        "tempi_mlp_num_outputs": (_I, []),
+        "tempi_mlp_num_params": (_I, []),
+        "tempi_mlp_features": (_I, [sb_p, _I64, _P, _P, _I, _P]),
+        "tempi_mlp_grad": (_I, [_P, _P, _P, _P, _I, _P, _P]),
+        "tempi_mlp_train": (_I, [_P, _P, _P, _I64, _I, ctypes.c_float, ctypes.c_uint64, _P, _P]),
+        "tempi_mlp_predict": (_I, [_P, _P, _P, _I64, _P, _P]),
+        "tempi_mlp_save": (_I, [ctypes.c_char_p, _P, _P]),
+        "tempi_mlp_load": (_I, [ctypes.c_char_p]),
+        "tempi_mlp_loaded": (_I, []),
     }
     for name, (res, args) in protos.items():
         fn = getattr(L, name)

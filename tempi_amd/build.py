@@ -28,6 +28,7 @@ SOURCES = [
     "kernels.hip",
     "autotune.hip",
     "batch.hip",
+    "mlp.hip",
 ]
 
 ARCH = os.environ.get("TEMPI_ARCH", "gfx950")

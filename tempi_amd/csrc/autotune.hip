@@ -24,13 +24,12 @@ const int kGridCaps[] = {0, 2, 4, 8, 32};
 const int kNumGridCaps = sizeof(kGridCaps) / sizeof(kGridCaps[0]);
 
 namespace {
-
 std::mutex g_tune_mu;
 std::map<std::string, Choice> g_tune_cache;
+} // namespace
 
-// the shape and alignment facts a copy's speed depends on
-std::string shape_key(const Geom &g, const char *strided, const char *packed,
-                      int dir) {
+std::string shape_key_of(const Geom &g, const char *strided,
+                         const char *packed, int dir) {
   std::ostringstream os;
   os << dir << ':' << g.L << ':' << g.nrd;
   for (int k = 0; k < g.nrd; ++k) {
@@ -40,8 +39,6 @@ std::string shape_key(const Geom &g, const char *strided, const char *packed,
      << ((uintptr_t)packed & 15);
   return os.str();
 }
-
-} // namespace
 
 int time_config(const Geom &g, const char *strided, char *packed, int dir,
                 int variant, int max_blocks, int reps, hipStream_t stream,
@@ -107,10 +104,19 @@ Choice choose(const Geom &g, const char *strided, char *packed, int dir,
               int method, hipStream_t stream) {
   Choice c;
   c.variant = heuristic_variant(g, strided, packed, dir);
+  if (method == TEMPI_SELECT_MODEL ||
+      (method == TEMPI_SELECT_DEFAULT && model_available())) {
+    const int v =
+        model_variant(g, strided, packed, dir, shape_key_of(g, strided, packed, dir));
+    if (v >= 0) {
+      c.variant = v;
+    }
+    return c;
+  }
   if (method != TEMPI_SELECT_AUTOTUNE || !stream) {
     return c;
   }
-  const std::string key = shape_key(g, strided, packed, dir);
+  const std::string key = shape_key_of(g, strided, packed, dir);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);

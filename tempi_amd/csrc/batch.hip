@@ -5,18 +5,18 @@
 // request once, on the host, into the same per-copy parameters the single
 // kernels take, uploads them as a device-resident table, and splits each
 // request into tiles of one workgroup pass (256 threads, four loads in
-// flight per lane: 16 KiB for 16-byte words).  A persistent grid (a few
-// workgroups per CU) drains the tile queue: each workgroup takes the next
-// tile index with one atomic add (fetched one tile ahead so the atomic's
-// latency overlaps the copy), reads the owning request from a tile-to-request
-// table, and runs that request's kernel variant on the tile's units.  Heterogeneous shapes -- 12-B halo rows next to 2-KiB face
-// rows -- share one launch.
+// flight per lane: 16 KiB for 16-byte words).  A persistent grid (up to 8
+// workgroups per CU) drains the tile queue with a static, XCD-aware
+// assignment; each tile's request comes from a tile-to-request table, and a
+// switch runs that request's kernel variant.  Heterogeneous shapes -- 12-B
+// halo rows next to 2-KiB face rows -- share one launch.
 //
-// The queue counter is reset by the last workgroup to leave, so the launch
-// needs no memset before it and can be replayed from a captured hipGraph.
-// Launches of one batch must not overlap in time (they share the counter);
-// requests inside a batch must not write bytes another request reads or
-// writes.
+// A first version handed tiles out with one atomic counter; every tile then
+// paid for a device-scope atomic serialized with all the others (about 30 ns
+// each on MI355X), which dominated small and large batches alike.  The
+// static assignment has no shared state, so launches need no reset and
+// replay from a captured hipGraph unchanged.  Requests inside a batch must
+// not write bytes another request reads or writes.
 #include "copy_core.hpp"
 
 #include <hip/hip_runtime.h>
@@ -35,10 +35,7 @@ struct BatchItem {
   uint32_t units_per_tile; // units of one block-pass: groups per block * K
 };
 
-struct BatchCtl {
-  uint32_t next; // next tile to hand out
-  uint32_t done; // workgroups that have left
-};
+constexpr uint32_t kXcds = 8;
 
 // one block-pass over units [u0, u0 + GROUPS*K) of a request
 template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
@@ -71,40 +68,37 @@ __device__ __forceinline__ void run_tile(int kind, const KParams &p,
   }
 }
 
+// Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
+// XCD b % 8).  With `xcd_split`, XCD x's workgroups take the contiguous tile
+// range [x*ntiles/8, (x+1)*ntiles/8), so neighbouring tiles -- whose rows
+// share partial cache lines -- stay in one XCD's L2; otherwise workgroups
+// simply stride over all tiles.  The assignment is static: no atomics, no
+// state carried between launches.
 __global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(5))) k_batch(const BatchItem *__restrict__ items,
+    __attribute__((amdgpu_waves_per_eu(5)))
+    k_batch(const BatchItem *__restrict__ items,
             const uint32_t *__restrict__ tile_begin,
             const uint32_t *__restrict__ tile_item, uint32_t ntiles,
-            BatchCtl *ctl) {
-  __shared__ uint32_t s_next;
-  if (threadIdx.x == 0) {
-    s_next = atomicAdd(&ctl->next, 1u);
+            int xcd_split) {
+  uint32_t t, end, step;
+  if (xcd_split) {
+    const uint32_t xcd = blockIdx.x % kXcds;
+    step = gridDim.x / kXcds;
+    t = (uint32_t)((uint64_t)ntiles * xcd / kXcds) + blockIdx.x / kXcds;
+    end = (uint32_t)((uint64_t)ntiles * (xcd + 1) / kXcds);
+  } else {
+    step = gridDim.x;
+    t = blockIdx.x;
+    end = ntiles;
   }
-  __syncthreads();
-  uint32_t t = __builtin_amdgcn_readfirstlane(s_next);
-  __syncthreads();
-  while (t < ntiles) {
-    if (threadIdx.x == 0) {
-      s_next = atomicAdd(&ctl->next, 1u); // one tile ahead
-    }
-    // the request owning tile t and the tile's first unit
+  for (; t < end; t += step) {
+    // the request owning tile t and the tile's first unit; a uniform
+    // address, so the parameters are read with scalar loads in place (a
+    // private copy of the struct would live in scratch memory)
     const uint32_t r = __builtin_amdgcn_readfirstlane(tile_item[t]);
-    // read the request's parameters in place (uniform address: scalar
-    // loads); a private copy of the struct would live in scratch memory
     const BatchItem &it = items[r];
     const uint32_t u0 = (t - tile_begin[r]) * it.units_per_tile;
     run_tile(it.kind, it.p, u0);
-    __syncthreads();
-    t = __builtin_amdgcn_readfirstlane(s_next);
-    __syncthreads();
-  }
-  // the last workgroup out rearms the queue for the next launch
-  if (threadIdx.x == 0) {
-    __threadfence();
-    if (atomicAdd(&ctl->done, 1u) == gridDim.x - 1) {
-      atomicExch(&ctl->next, 0u);
-      atomicExch(&ctl->done, 0u);
-    }
   }
 }
 
@@ -120,11 +114,11 @@ struct tempi_batch {
   uint32_t ntiles = 0;
   int grid = 0;
   int n = 0; // items with work
-  char *dev = nullptr; // items, tile tables, ctl in one allocation
+  char *dev = nullptr; // items and tile tables in one allocation
   BatchItem *d_items = nullptr;
   uint32_t *d_tile_begin = nullptr; // first tile of each item
   uint32_t *d_tile_item = nullptr;  // item of each tile
-  BatchCtl *d_ctl = nullptr;
+  int xcd_split = 0;
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -136,7 +130,7 @@ static int batch_launch_on(tempi_batch *b, hipStream_t s) {
     return TEMPI_SUCCESS;
   }
   hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0, s, b->d_items,
-                     b->d_tile_begin, b->d_tile_item, b->ntiles, b->d_ctl);
+                     b->d_tile_begin, b->d_tile_item, b->ntiles, b->xcd_split);
   TEMPI_HIP_CHECK(hipGetLastError());
   return TEMPI_SUCCESS;
 }
@@ -214,13 +208,19 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   }
   b->n = (int)items.size();
   b->ntiles = (uint32_t)tiles;
-  b->grid = (int)std::max<int64_t>(
-      1, std::min<int64_t>((int64_t)tiles, (int64_t)device_cu_count() * 8));
+  // persistent grid: up to 8 workgroups per CU, a multiple of the XCD count
+  // when the tiles are split by XCD
+  const int64_t cap = (int64_t)device_cu_count() * 8;
+  if ((int64_t)tiles >= 8 * (int64_t)kXcds) {
+    b->xcd_split = 1;
+    b->grid = (int)(std::min<int64_t>((int64_t)tiles, cap) / kXcds * kXcds);
+  } else {
+    b->grid = (int)std::max<int64_t>(1, (int64_t)tiles);
+  }
   const size_t items_bytes = items.size() * sizeof(BatchItem);
   const size_t tb_off = (items_bytes + 255) / 256 * 256;
   const size_t ti_off = tb_off + (tile_begin.size() * 4 + 255) / 256 * 256;
-  const size_t ctl_off = ti_off + (tile_item.size() * 4 + 255) / 256 * 256;
-  const size_t total = ctl_off + 256;
+  const size_t total = ti_off + (tile_item.size() * 4 + 255) / 256 * 256 + 256;
   if (hipSuccess != hipMalloc(&b->dev, total)) {
     (void)hipGetLastError();
     delete b;
@@ -229,7 +229,6 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   b->d_items = reinterpret_cast<BatchItem *>(b->dev);
   b->d_tile_begin = reinterpret_cast<uint32_t *>(b->dev + tb_off);
   b->d_tile_item = reinterpret_cast<uint32_t *>(b->dev + ti_off);
-  b->d_ctl = reinterpret_cast<BatchCtl *>(b->dev + ctl_off);
   hipStream_t s = library_stream();
   std::vector<char> host(total, 0);
   if (!items.empty()) {
@@ -237,7 +236,7 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
     memcpy(host.data() + tb_off, tile_begin.data(), tile_begin.size() * 4);
     memcpy(host.data() + ti_off, tile_item.data(), tile_item.size() * 4);
   }
-  // one upload (the counter words are zero in `host`), outside any capture
+  // one upload, outside any capture
   if (hipSuccess != hipMemcpyAsync(b->dev, host.data(), total,
                                    hipMemcpyHostToDevice, s) ||
       hipSuccess != hipStreamSynchronize(s)) {
@@ -276,7 +275,7 @@ TEMPI_EXPORT int tempi_batch_graph_launch(tempi_batch *b, void *stream) {
                                           hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0,
                        b->capture_stream, b->d_items, b->d_tile_begin,
-                       b->d_tile_item, b->ntiles, b->d_ctl);
+                       b->d_tile_item, b->ntiles, b->xcd_split);
     TEMPI_HIP_CHECK(hipStreamEndCapture(b->capture_stream, &b->graph));
     TEMPI_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }

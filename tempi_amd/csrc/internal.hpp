@@ -184,6 +184,18 @@ int time_config(const Geom &g, const char *strided, char *packed, int dir,
                 int variant, int max_blocks, int reps, hipStream_t stream,
                 float *us);
 
+// the shape and alignment facts a copy's speed depends on (cache key)
+std::string shape_key_of(const Geom &g, const char *strided,
+                         const char *packed, int dir);
+
+// learned selector (mlp.hip)
+void copy_features(const Geom &g, const char *strided, const char *packed,
+                   int dir, float *f);
+bool model_available();
+int model_variant(const Geom &g, const char *strided, const char *packed,
+                  int dir, const std::string &key); // -1: no model
+void model_precommit(const tempi_strided_block &sb);
+
 // exhaustive search; fills best and, when given, the best time per variant
 int autotune(const Geom &g, const char *strided, char *packed, int dir,
              hipStream_t stream, Choice *best, float *per_variant_us);

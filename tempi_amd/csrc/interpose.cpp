@@ -125,6 +125,11 @@ TEMPI_EXPORT int MPI_Type_commit(MPI_Datatype *datatype) {
   rec.canon = canonicalize(*datatype);
   if (rec.canon.ok) {
     ++counters().commits_canonical;
+    if (env().select == TEMPI_SELECT_MODEL ||
+        (env().select == TEMPI_SELECT_DEFAULT && model_available())) {
+      // evaluate the learned selector for this type now, not in MPI_Pack
+      model_precommit(rec.canon.sb);
+    }
   } else {
     ++counters().commits_unsupported;
     TEMPI_LOG(LOG_DEBUG, "type %lld: library path (%s)",

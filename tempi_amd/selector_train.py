@@ -1,0 +1,272 @@
+"""Train the learned kernel-variant selector on this GPU.
+
+    python -m tempi_amd.selector_train --out tempi_amd/selector_mlp.txt \
+        --report report.json [--train 50000] [--heldout 10000]
+
+1. Generate seeded strided copies of the kinds the path sees: 2D/3D byte
+   vectors with block lengths 1 B - 1 KiB and strides up to 4 KiB past the
+   block at any base offset, halo faces/edges/corners of 3D subdomains, faces
+   of float grids, and random 1D/2D/3D blocks; packs and unpacks.
+2. Time every kernel variant on each with the autotuner's timer
+   (tempi_time_variant: HIP events around back-to-back launches).
+3. Train the MLP (tempi_mlp_train: the library's HIP forward / backward /
+   Adam kernels) to predict each variant's log2 time relative to the mean
+   over variants; features are normalized by the training set's mean / std.
+4. On the held-out copies, report the regret of the model's choice, of the
+   closed-form rule, against the exhaustive autotuner's best, and the packed
+   GB/s each choice achieves; write the model file the library loads.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import random
+import sys
+import time
+
+import numpy as np
+
+import tempi_amd as ta
+
+SRC_BYTES = 3 << 30
+DST_BYTES = 64 << 20
+
+
+def _vector_case(rng, max_packed):
+    bls = [1, 3, 4, 8, 12, 16, 24, 32, 64, 128, 256, 512, 1024]
+    while True:
+        bl = rng.choice(bls)
+        rows = 1 << rng.randint(4, 20)
+        stride = bl + rng.randint(0, 4096)
+        if rng.random() < 0.5:
+            counts, strides = [bl, rows], [1, stride]
+            span = (rows - 1) * stride + bl
+        else:
+            planes = rng.randint(2, 8)
+            rows = max(1, rows // planes)
+            s2 = rows * stride + rng.randint(0, 4096)
+            counts, strides = [bl, rows, planes], [1, stride, s2]
+            span = (planes - 1) * s2 + (rows - 1) * stride + bl
+        packed = int(np.prod(counts))
+        if 1024 <= packed <= max_packed and span < SRC_BYTES - 64:
+            return ta.StridedBlock.make(0, span, counts, strides), rng.randint(0, 15), span
+
+
+def _halo_case(rng, max_packed):
+    while True:
+        n = [rng.randint(16, 256) for _ in range(3)]
+        r = rng.randint(1, 4)
+        q = rng.choice([4, 8])
+        d = [rng.choice([-1, 0, 1]) for _ in range(3)]
+        if d == [0, 0, 0]:
+            continue
+        ext = [n[i] if d[i] == 0 else r for i in range(3)]
+        pos = [r if d[i] <= 0 else n[i] for i in range(3)]
+        pitch = (n[0] + 2 * r) * q
+        ysz = n[1] + 2 * r
+        start = (pos[2] * ysz + pos[1]) * pitch + pos[0] * q
+        counts = [ext[0] * q, ext[1], ext[2]]
+        strides = [1, pitch, pitch * ysz]
+        span = (n[2] + 2 * r) * ysz * pitch
+        if counts[0] * ext[1] * ext[2] <= max_packed and span < SRC_BYTES:
+            return ta.StridedBlock.make(start, span, counts, strides), 0, span
+
+
+def _grid_face_case(rng, max_packed):
+    while True:
+        n = rng.choice([32, 64, 128, 256, 384, 512])
+        g = rng.randint(1, 4)
+        e = rng.choice([4, 8])
+        full = n + 2 * g
+        face = rng.choice("xyz")
+        sub = {"x": [n, n, g], "y": [n, g, n], "z": [g, n, n]}[face]
+        start = ((g * full + g) * full + g) * e
+        counts = [sub[2] * e, sub[1], sub[0]]
+        strides = [1, full * e, full * full * e]
+        span = full ** 3 * e
+        if int(np.prod(counts)) <= max_packed and span < SRC_BYTES:
+            return ta.StridedBlock.make(start, span, counts, strides), 0, span
+
+
+def _random_case(rng, max_packed):
+    while True:
+        target = int(math.exp(rng.uniform(math.log(64), math.log(max_packed))))
+        nd = rng.choice([1, 2, 3])
+        if nd == 1:
+            counts, strides = [target], [1]
+        else:
+            bl = min(rng.choice([4, 8, 12, 16, 32, 64, 128, 256, 512, 1024, 2048]), target)
+            rows = max(1, target // bl)
+            s1 = bl + rng.randint(0, 512)
+            if nd == 2:
+                counts, strides = [bl, rows], [1, s1]
+            else:
+                r1 = max(1, int(math.sqrt(rows)))
+                r2 = max(1, rows // r1)
+                counts, strides = [bl, r1, r2], [1, s1, s1 * r1 + rng.randint(0, 4096)]
+        span = sum((c - 1) * s for c, s in zip(counts, strides)) + counts[0]
+        if span < SRC_BYTES - 64:
+            return ta.StridedBlock.make(0, span, counts, strides), rng.randint(0, 3) * 4, span
+
+
+def generate(n, seed, max_packed):
+    rng = random.Random(seed)
+    cases = []
+    for _ in range(n):
+        u = rng.random()
+        if u < 0.4:
+            sb, base, span = _vector_case(rng, max_packed)
+        elif u < 0.6:
+            sb, base, span = _halo_case(rng, max_packed)
+        elif u < 0.8:
+            sb, base, span = _grid_face_case(rng, max_packed)
+        else:
+            sb, base, span = _random_case(rng, max_packed)
+        direction = rng.randint(0, 1)
+        ppos = rng.choice([0, 0, 0, 4, 8, rng.randint(0, 63)])
+        cases.append((sb, base, direction, ppos))
+    return cases
+
+
+def time_cases(cases, src, dst, log_every=2000, tag=""):
+    L = ta.lib()
+    nv = ta.num_variants()
+    nf = L.tempi_mlp_num_features()
+    X = np.zeros((len(cases), nf), np.float32)
+    T = np.zeros((len(cases), nv), np.float64)
+    heur = np.zeros(len(cases), np.int32)
+    nbytes = np.zeros(len(cases), np.float64)
+    f = (ctypes.c_float * nf)()
+    us = ctypes.c_float()
+    t0 = time.time()
+    for i, (sb, base, direction, ppos) in enumerate(cases):
+        strided, packed = src.addr(base), dst.addr(ppos)
+        size = sb.size()
+        reps = 5 if size < (1 << 20) else (3 if size < (8 << 20) else 2)
+        ta.check(L.tempi_mlp_features(ctypes.byref(sb), 1, strided, packed, direction, f))
+        X[i] = np.frombuffer(f, np.float32)
+        for v in range(nv):
+            ta.check(L.tempi_time_variant(ctypes.byref(sb), 1, strided, packed, direction, v, 0,
+                                          reps, ctypes.byref(us)), "time_variant")
+            T[i, v] = max(float(us.value), 1e-3)
+        heur[i] = ta.select_variant(sb, 1, strided, packed, direction, ta.SELECT_HEURISTIC)
+        nbytes[i] = size
+        if log_every and (i + 1) % log_every == 0:
+            print(f"[selector_train] {tag} timed {i + 1}/{len(cases)} in {time.time() - t0:.0f} s",
+                  flush=True)
+    return X, T, heur, nbytes
+
+
+def regret_stats(T, choice):
+    best = T.min(1)
+    r = T[np.arange(len(T)), choice] / best - 1.0
+    return {"mean": float(r.mean()), "median": float(np.median(r)),
+            "p90": float(np.quantile(r, 0.9)), "within_5pct": float((r <= 0.05).mean())}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=50000)
+    ap.add_argument("--heldout", type=int, default=10000)
+    ap.add_argument("--max-packed", type=int, default=32 << 20)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=5000)
+    ap.add_argument("--heldout-seed", type=int, default=900000)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--report", required=True)
+    ap.add_argument("--dataset", default=None, help="npz to save (or reuse) the timings")
+    args = ap.parse_args(argv)
+
+    L = ta.lib()
+    assert ta.device_count() > 0, "no GPU"
+    nv = ta.num_variants()
+    data = None
+    if args.dataset and os.path.exists(args.dataset):
+        data = dict(np.load(args.dataset))
+        print(f"[selector_train] reusing {args.dataset}", flush=True)
+    if data is None:
+        src = ta.DeviceBuffer(SRC_BYTES)
+        src.fill(1)
+        dst = ta.DeviceBuffer(DST_BYTES + 4096)
+        dst.fill(2)
+        t0 = time.time()
+        tr = time_cases(generate(args.train, args.seed, args.max_packed), src, dst, tag="train")
+        ho = time_cases(generate(args.heldout, args.heldout_seed, args.max_packed), src, dst,
+                        tag="heldout")
+        data = {"Xtr": tr[0], "Ttr": tr[1], "Htr": tr[2], "Btr": tr[3],
+                "Xho": ho[0], "Tho": ho[1], "Hho": ho[2], "Bho": ho[3]}
+        print(f"[selector_train] timing took {time.time() - t0:.0f} s", flush=True)
+        if args.dataset:
+            np.savez_compressed(args.dataset, **data)
+        src.free()
+        dst.free()
+
+    Xtr, Ttr = data["Xtr"], data["Ttr"]
+    mean = Xtr.mean(0)
+    std = Xtr.std(0)
+    inv = 1.0 / np.maximum(std, 1e-3)
+    norm = np.concatenate([mean, inv]).astype(np.float32)
+    Xn = ((Xtr - mean) * inv).astype(np.float32)
+    lt = np.log2(Ttr)
+    target = (lt - lt.mean(1, keepdims=True)).astype(np.float32)
+
+    rng = np.random.default_rng(args.seed)
+    parts = []
+    for s in [(16, 64), (64,), (64, 64), (64,), (64, nv), (nv,)]:
+        parts.append(rng.standard_normal(s) * math.sqrt(2.0 / s[0]) if len(s) == 2
+                     else np.zeros(s))
+    params = np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+    assert params.size == L.tempi_mlp_num_params()
+    n = len(Xn)
+    t0 = time.time()
+    losses = []
+    for epochs, lr in ((args.epochs * 2 // 3, args.lr), (args.epochs - args.epochs * 2 // 3,
+                                                          args.lr / 4)):
+        if epochs <= 0:
+            continue
+        el = np.zeros(epochs, np.float32)
+        ta.check(L.tempi_mlp_train(Xn.ctypes.data, target.ctypes.data, None, n, epochs,
+                                   ctypes.c_float(lr), args.seed, params.ctypes.data,
+                                   el.ctypes.data), "tempi_mlp_train")
+        losses += el.tolist()
+        print(f"[selector_train] {epochs} epochs at lr {lr}: loss {el[0]:.4f} -> {el[-1]:.4f}",
+              flush=True)
+    train_s = time.time() - t0
+
+    def predict(X):
+        best = np.zeros(len(X), np.int32)
+        ta.check(L.tempi_mlp_predict(params.ctypes.data, norm.ctypes.data,
+                                     np.ascontiguousarray(X, np.float32).ctypes.data, len(X),
+                                     None, best.ctypes.data), "tempi_mlp_predict")
+        return best
+
+    ta.check(L.tempi_mlp_save(args.out.encode(), params.ctypes.data, norm.ctypes.data), "save")
+    report = {"train_samples": int(n), "heldout_samples": int(len(data["Xho"])),
+              "variants": nv, "epochs": args.epochs, "train_seconds": round(train_s, 2),
+              "loss_first_last": [losses[0], losses[-1]]}
+    for split, X, T, H, B in (("train", data["Xtr"], data["Ttr"], data["Htr"], data["Btr"]),
+                              ("heldout", data["Xho"], data["Tho"], data["Hho"], data["Bho"])):
+        pm = predict(X)
+        best = T.argmin(1)
+        idx = np.arange(len(T))
+        report[split] = {
+            "model_regret": regret_stats(T, pm),
+            "heuristic_regret": regret_stats(T, H),
+            "model_picks_best": float((pm == best).mean()),
+            # packed GB/s summed over the set: bytes / time of the chosen variant
+            "gbs_model": float(B.sum() / (T[idx, pm].sum() * 1e-6) / 1e9),
+            "gbs_heuristic": float(B.sum() / (T[idx, H].sum() * 1e-6) / 1e9),
+            "gbs_autotuner_best": float(B.sum() / (T[idx, best].sum() * 1e-6) / 1e9),
+        }
+    with open(args.report, "w") as fh:
+        json.dump(report, fh, indent=1)
+    print(json.dumps(report["heldout"]), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

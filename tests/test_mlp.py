@@ -1,0 +1,122 @@
+"""The variant-selector MLP on the CPU: the oracle's gradient against finite
+differences (and a planted mistake against the same check), the feature
+extractor against its definition, and the model file format."""
+import ctypes
+import math
+import os
+
+import numpy as np
+import pytest
+
+import tempi_amd as ta
+from oracle import mlp_ref
+
+OUT = 13
+
+
+def fd_grad(params, x, t, m, idx, h=1e-6):
+    g = []
+    for i in idx:
+        p1 = params.astype(np.float64).copy()
+        p2 = p1.copy()
+        p1[i] += h
+        p2[i] -= h
+        l1, _ = mlp_ref.loss_and_grad(p1, x, t, m, OUT)
+        l2, _ = mlp_ref.loss_and_grad(p2, x, t, m, OUT)
+        g.append((l1 - l2) / (2 * h))
+    return np.array(g)
+
+
+def sample_problem(seed, n=9):
+    rng = np.random.default_rng(seed)
+    params = mlp_ref.init_params(OUT, seed).astype(np.float64)
+    params += rng.standard_normal(params.size) * 0.05  # nonzero biases
+    x = rng.standard_normal((n, 16))
+    t = rng.standard_normal((n, OUT))
+    m = (rng.random((n, OUT)) > 0.2).astype(np.float64)
+    return params, x, t, m
+
+
+def test_oracle_gradient_matches_finite_differences():
+    params, x, t, m = sample_problem(1)
+    _, g = mlp_ref.loss_and_grad(params, x, t, m, OUT)
+    rng = np.random.default_rng(2)
+    n = params.size
+    # every parameter block: W1, b1, W2, b2, W3, b3
+    offs = np.cumsum([0] + [int(np.prod(s)) for s in mlp_ref.shapes(OUT)])
+    idx = np.concatenate([rng.integers(offs[k], offs[k + 1], 12) for k in range(6)])
+    assert offs[-1] == n == ta.lib().tempi_mlp_num_params()
+    fd = fd_grad(params, x, t, m, idx)
+    np.testing.assert_allclose(g[idx], fd, rtol=1e-5, atol=1e-8)
+
+
+def test_planted_gradient_mistake_is_caught():
+    """A backward pass that forgets the ReLU mask of the second layer must
+    fail the finite-difference check."""
+    params, x, t, m = sample_problem(3)
+
+    def wrong(params):
+        W1, b1, W2, b2, W3, b3 = mlp_ref.unpack(params, OUT)
+        y, (xx, h1, h2) = mlp_ref.forward(params, x, OUT)
+        dy = 2.0 * m * (y - t) / m.sum()
+        d2 = dy @ W3.T  # mask dropped
+        return h1.T @ d2
+
+    offs = np.cumsum([0] + [int(np.prod(s)) for s in mlp_ref.shapes(OUT)])
+    idx = np.arange(offs[2], offs[3])
+    fd = fd_grad(params, x, t, m, idx).reshape(64, 64)
+    assert not np.allclose(wrong(params), fd, rtol=1e-4, atol=1e-7)
+
+
+def test_adam_step_closed_form():
+    # first step of Adam moves every parameter by lr * sign(g) (bias-corrected)
+    p = np.array([1.0, -2.0, 0.5])
+    g = np.array([0.3, -4.0, 1e-3])
+    p2, m1, m2 = mlp_ref.adam_step(p, np.zeros(3), np.zeros(3), g, 0.01, 1)
+    np.testing.assert_allclose(p - p2, 0.01 * np.sign(g), rtol=1e-4)
+
+
+def test_features_follow_their_definition():
+    L = ta.lib()
+    f = (ctypes.c_float * 16)()
+    # x face of the float grid at its real addresses' alignment
+    sb = ta.StridedBlock.make(3226116, 555967328, [12, 512, 512], [1, 2072, 1073296])
+    assert L.tempi_mlp_features(ctypes.byref(sb), 1, 1 << 20, 1 << 21, 0, f) == 0
+    exp = [math.log2(12), math.log2(512 * 512), math.log2(12 * 512 * 512), 2,
+           math.log2(512), math.log2(2072), math.log2(512), math.log2(1073296), 0,
+           2,  # 3226116 = 4 * odd
+           4, 3,  # strides 2072 = 8 * 259
+           2, 0, 1, math.log2(2072 / 12)]
+    np.testing.assert_allclose(list(f), exp, rtol=1e-5)
+    # contiguous type: one row, unpack
+    sb = ta.StridedBlock.make(0, 4096, [4096], [1])
+    assert L.tempi_mlp_features(ctypes.byref(sb), 3, 1 << 20, (1 << 20) + 3, 1, f) == 0
+    exp = [math.log2(3 * 4096), 0, math.log2(3 * 4096), 0, 0, 0, 0, 0, 0, 4, 0, 4, 4, 1, 0, 0]
+    np.testing.assert_allclose(list(f), exp, rtol=1e-5)
+
+
+def test_model_file_round_trip_and_rejects(tmp_path):
+    L = ta.lib()
+    n = L.tempi_mlp_num_params()
+    params = mlp_ref.init_params(OUT, 5)
+    norm = np.concatenate([np.arange(16, dtype=np.float32), np.ones(16, np.float32)])
+    path = str(tmp_path / "m.txt")
+    assert L.tempi_mlp_save(path.encode(), params.ctypes.data, norm.ctypes.data) == 0
+    text = open(path).read().split()
+    assert text[:2] == ["tempi-mlp", "1"]
+    vals = np.array([float(v) for v in text[text.index("params") + 1:]], np.float32)
+    assert vals.size == n and (vals == params).all()
+    assert L.tempi_mlp_load(path.encode()) == 0 and L.tempi_mlp_loaded() == 1
+    bad = tmp_path / "bad.txt"
+    bad.write_text(open(path).read().replace("out 13", "out 12"))
+    assert L.tempi_mlp_load(str(bad).encode()) == ta.TEMPI_ERR_IO
+    assert L.tempi_mlp_loaded() == 0
+    trunc = tmp_path / "trunc.txt"
+    trunc.write_text(open(path).read()[:2000])
+    assert L.tempi_mlp_load(str(trunc).encode()) == ta.TEMPI_ERR_IO
+    assert L.tempi_mlp_load(b"/nonexistent/model.txt") == ta.TEMPI_ERR_IO
+    # without a model the selection is the closed-form rule
+    sb = ta.StridedBlock.make(0, 524032, [256, 1024], [1, 512])
+    assert ta.select_variant(sb, 1, 1 << 20, 1 << 20, method=ta.SELECT_MODEL) == \
+        ta.select_variant(sb, 1, 1 << 20, 1 << 20, method=ta.SELECT_HEURISTIC)
+    L.tempi_mlp_load(None)  # back to the default model, if one is installed
