@@ -15,7 +15,7 @@ roofline the batch kernel's algorithmic HBM bytes per launch (2 x packed:
          region, against the 8 TB/s HBM3E peak
 extras   the same faces as six separate kernels (the MPI_Pack kernels) and
          as synchronous MPI_Pack calls; cold-cache kernel times (caches
-         scrubbed by a 1 GiB write before each step); the halo batch (26
+         swept by a read-only 1 GiB pass before each step); the halo batch (26
          halos of 64 subdomains of 128^3 float64, radius 3) and 4096 random
          requests, batched vs one call per request; the 2D vector case.
 cpu_baseline  the oracle's single-threaded C gather/scatter (oracle/oracle.c)
@@ -176,12 +176,12 @@ def main():
                             "hbm_gbs": round(2 * f["size"] / us / 1e3, 1)})
     extras["single_kernels_warm"] = kernels
 
-    # cold caches: a 1 GiB write before each step evicts L2 and the MALL
-    scrub = ta.DeviceBuffer(1 << 30)
+    # cold caches: a read-only 1 GiB sweep before each step leaves L2 and the
+    # Infinity Cache full of clean, unrelated lines
     ev = [ta.Event() for _ in range(4)]
     cold = {"pack_batch_us": [], "unpack_batch_us": []}
     for _ in range(20):
-        scrub.fill(1)
+        ta.flush_caches()
         ev[0].record(stream)
         pack_batch.launch(stream)
         ev[1].record(stream)
@@ -195,7 +195,6 @@ def main():
         k: {"median_us": round(float(np.median(v)), 3),
             "hbm_gbs": round(2 * half / float(np.median(v)) / 1e3, 1)}
         for k, v in cold.items()}
-    del scrub
 
     # synchronous MPI_Pack / MPI_Unpack calls (interposed), wall clock
     call_us = {}

@@ -165,6 +165,10 @@ int tempi_autotune(const tempi_strided_block *sb, int64_t incount,
                    const void *strided, void *packed, int direction,
                    int *best_variant, int *best_grid_cap,
                    float *per_variant_us);
+/* Read `bytes` (<= 0: 1 GiB) of a library-owned buffer so that L2 and the
+   Infinity Cache hold only clean, unrelated lines; synchronous.  For
+   cold-cache timing. */
+int tempi_flush_caches(int64_t bytes);
 /* Launch latency of an empty kernel: host-side enqueue cost and device
    time per launch (reference: src/internal/measure_system.cu:36-64). */
 int tempi_launch_latency_us(int reps, float *host_us, float *device_us);

@@ -174,6 +174,7 @@ def _prototypes(L):
                                     ctypes.POINTER(ctypes.c_float)]),
         "tempi_autotune": (_I, [sb_p, _I64, _P, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                 ctypes.POINTER(ctypes.c_float)]),
+        "tempi_flush_caches": (_I, [_I64]),
         "tempi_launch_latency_us": (_I, [_I, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(ctypes.c_float)]),
         "tempi_batch_create": (_I, [ctypes.POINTER(BatchRequest), _I64, _I, ctypes.POINTER(_P)]),
@@ -377,6 +378,11 @@ def autotune(sb: StridedBlock, incount: int, strided: int, packed: int, directio
     check(lib().tempi_autotune(ctypes.byref(sb), incount, strided, packed, direction,
                                ctypes.byref(v), ctypes.byref(c), per), "autotune")
     return v.value, c.value, list(per)
+
+
+def flush_caches(nbytes: int = 0):
+    """Evict L2 / Infinity Cache with a read-only sweep (for cold timings)."""
+    check(lib().tempi_flush_caches(nbytes), "flush_caches")
 
 
 def launch_latency_us(reps: int = 1000):

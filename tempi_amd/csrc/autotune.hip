@@ -141,6 +141,38 @@ __global__ void k_empty(int *p) {
   }
 }
 
+// Reads one dword per 64 B of `buf` and writes one word per workgroup: the
+// caches end up full of clean lines of `buf`, so a kernel timed after it
+// starts from HBM without write-backs of someone else's dirty data.
+__global__ void __launch_bounds__(256) k_touch(const uint32_t *__restrict__ buf,
+                                               uint64_t words, uint32_t *sink) {
+  uint32_t acc = 0;
+  for (uint64_t i = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 16; i < words;
+       i += (uint64_t)gridDim.x * 256 * 16) {
+    acc ^= buf[i];
+  }
+  __shared__ uint32_t s;
+  if (threadIdx.x == 0) {
+    s = 0;
+  }
+  __syncthreads();
+  atomicXor(&s, acc);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sink[blockIdx.x] = s;
+  }
+}
+
+struct FlushBuf {
+  std::mutex mu;
+  char *buf = nullptr;
+  int64_t bytes = 0;
+};
+FlushBuf &flush_buf() {
+  static FlushBuf f;
+  return f;
+}
+
 } // namespace tempi
 
 using namespace tempi;
@@ -212,6 +244,39 @@ TEMPI_EXPORT int tempi_autotune(const tempi_strided_block *sb, int64_t incount,
       }
     }
   }
+  return TEMPI_SUCCESS;
+}
+
+// Evict L2 and the Infinity Cache by reading `bytes` (default 1 GiB) of a
+// library-owned buffer; synchronous.
+TEMPI_EXPORT int tempi_flush_caches(int64_t bytes) {
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  if (bytes <= 0) {
+    bytes = 1ll << 30;
+  }
+  FlushBuf &f = flush_buf();
+  std::lock_guard<std::mutex> lk(f.mu);
+  const int blocks = 2048;
+  if (f.bytes < bytes) {
+    if (f.buf) {
+      TEMPI_HIP_CHECK(hipFree(f.buf));
+      f.buf = nullptr;
+      f.bytes = 0;
+    }
+    TEMPI_HIP_CHECK(hipMalloc(&f.buf, (size_t)bytes + blocks * 4));
+    TEMPI_HIP_CHECK(hipMemset(f.buf, 1, (size_t)bytes));
+    f.bytes = bytes;
+  }
+  hipStream_t s = library_stream();
+  hipLaunchKernelGGL(k_touch, dim3(blocks), dim3(256), 0, s,
+                     reinterpret_cast<const uint32_t *>(f.buf),
+                     (uint64_t)bytes / 4,
+                     reinterpret_cast<uint32_t *>(f.buf + f.bytes));
+  TEMPI_HIP_CHECK(hipGetLastError());
+  TEMPI_HIP_CHECK(hipStreamSynchronize(s));
   return TEMPI_SUCCESS;
 }
 

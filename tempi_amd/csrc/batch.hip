@@ -68,36 +68,44 @@ __device__ __forceinline__ void run_tile(int kind, const KParams &p,
   }
 }
 
+// One tile: the request it belongs to and its first unit.
+struct Tile {
+  uint32_t item;
+  uint32_t u0;
+};
+
 // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
-// XCD b % 8).  With `xcd_split`, XCD x's workgroups take the contiguous tile
-// range [x*ntiles/8, (x+1)*ntiles/8), so neighbouring tiles -- whose rows
-// share partial cache lines -- stay in one XCD's L2; otherwise workgroups
-// simply stride over all tiles.  The assignment is static: no atomics, no
-// state carried between launches.
+// XCD b % 8).  The host orders the tile table so that XCD x owns the
+// contiguous range [xcd_begin[x], xcd_begin[x+1]), which holds an eighth of
+// every request's tiles: each XCD gets an equal share of each shape (costly
+// 12-B-row tiles and cheap 2-KiB-row tiles alike), and neighbouring tiles --
+// whose rows can share partial cache lines -- stay in one XCD's L2.  The
+// assignment is static: no atomics, no state carried between launches.
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(5)))
     k_batch(const BatchItem *__restrict__ items,
-            const uint32_t *__restrict__ tile_begin,
-            const uint32_t *__restrict__ tile_item, uint32_t ntiles,
+            const Tile *__restrict__ tiles,
+            const uint32_t *__restrict__ xcd_begin, uint32_t ntiles,
             int xcd_split) {
   uint32_t t, end, step;
   if (xcd_split) {
     const uint32_t xcd = blockIdx.x % kXcds;
     step = gridDim.x / kXcds;
-    t = (uint32_t)((uint64_t)ntiles * xcd / kXcds) + blockIdx.x / kXcds;
-    end = (uint32_t)((uint64_t)ntiles * (xcd + 1) / kXcds);
+    t = xcd_begin[xcd] + blockIdx.x / kXcds;
+    end = xcd_begin[xcd + 1];
   } else {
     step = gridDim.x;
     t = blockIdx.x;
     end = ntiles;
   }
   for (; t < end; t += step) {
-    // the request owning tile t and the tile's first unit; a uniform
-    // address, so the parameters are read with scalar loads in place (a
-    // private copy of the struct would live in scratch memory)
-    const uint32_t r = __builtin_amdgcn_readfirstlane(tile_item[t]);
+    // uniform addresses: the tile and its request's parameters are read
+    // with scalar loads in place (a private copy of the parameter struct
+    // would live in scratch memory)
+    const Tile tl = tiles[t];
+    const uint32_t r = __builtin_amdgcn_readfirstlane(tl.item);
+    const uint32_t u0 = __builtin_amdgcn_readfirstlane(tl.u0);
     const BatchItem &it = items[r];
-    const uint32_t u0 = (t - tile_begin[r]) * it.units_per_tile;
     run_tile(it.kind, it.p, u0);
   }
 }
@@ -116,8 +124,8 @@ struct tempi_batch {
   int n = 0; // items with work
   char *dev = nullptr; // items and tile tables in one allocation
   BatchItem *d_items = nullptr;
-  uint32_t *d_tile_begin = nullptr; // first tile of each item
-  uint32_t *d_tile_item = nullptr;  // item of each tile
+  Tile *d_tiles = nullptr;          // tile records, grouped by XCD
+  uint32_t *d_xcd_begin = nullptr;  // first tile of each XCD (kXcds + 1)
   int xcd_split = 0;
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -130,7 +138,7 @@ static int batch_launch_on(tempi_batch *b, hipStream_t s) {
     return TEMPI_SUCCESS;
   }
   hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0, s, b->d_items,
-                     b->d_tile_begin, b->d_tile_item, b->ntiles, b->xcd_split);
+                     b->d_tiles, b->d_xcd_begin, b->ntiles, b->xcd_split);
   TEMPI_HIP_CHECK(hipGetLastError());
   return TEMPI_SUCCESS;
 }
@@ -148,8 +156,7 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   tempi_batch *b = new tempi_batch();
   b->requests = n;
   std::vector<BatchItem> items;
-  std::vector<uint32_t> tile_begin;
-  std::vector<uint32_t> tile_item;
+  std::vector<uint32_t> item_tiles; // tiles of each item
   uint64_t tiles = 0;
   for (int64_t i = 0; i < n; ++i) {
     const tempi_batch_request &r = reqs[i];
@@ -196,13 +203,12 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
     it.units_per_tile = (uint32_t)(kBlock / v.P) * units_in_flight(v.U);
     const uint64_t t = ((uint64_t)it.p.units + it.units_per_tile - 1) /
                        it.units_per_tile;
-    tile_begin.push_back((uint32_t)tiles);
     tiles += t;
     if (tiles >= (1ull << 31)) {
       delete b;
       return set_error(TEMPI_ERR_UNSUPPORTED, "batch exceeds 2^31 tiles");
     }
-    tile_item.insert(tile_item.end(), t, (uint32_t)items.size());
+    item_tiles.push_back((uint32_t)t);
     items.push_back(it);
     b->bytes += g.L * g.rows;
   }
@@ -217,25 +223,43 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   } else {
     b->grid = (int)std::max<int64_t>(1, (int64_t)tiles);
   }
+  // tile records grouped by XCD: XCD x takes the x-th eighth of every
+  // item's tiles (a single group when the grid is not split)
+  const int groups = b->xcd_split ? (int)kXcds : 1;
+  std::vector<Tile> tile_list;
+  tile_list.reserve(tiles);
+  std::vector<uint32_t> xcd_begin(kXcds + 1, (uint32_t)tiles);
+  for (int x = 0; x < groups; ++x) {
+    xcd_begin[x] = (uint32_t)tile_list.size();
+    for (size_t r = 0; r < items.size(); ++r) {
+      const uint64_t n_r = item_tiles[r];
+      const uint64_t lo = n_r * x / groups, hi = n_r * (x + 1) / groups;
+      for (uint64_t l = lo; l < hi; ++l) {
+        tile_list.push_back(
+            Tile{(uint32_t)r, (uint32_t)(l * items[r].units_per_tile)});
+      }
+    }
+  }
   const size_t items_bytes = items.size() * sizeof(BatchItem);
-  const size_t tb_off = (items_bytes + 255) / 256 * 256;
-  const size_t ti_off = tb_off + (tile_begin.size() * 4 + 255) / 256 * 256;
-  const size_t total = ti_off + (tile_item.size() * 4 + 255) / 256 * 256 + 256;
+  const size_t tl_off = (items_bytes + 255) / 256 * 256;
+  const size_t xb_off = tl_off + (tile_list.size() * sizeof(Tile) + 255) / 256 * 256;
+  const size_t total = xb_off + 256;
   if (hipSuccess != hipMalloc(&b->dev, total)) {
     (void)hipGetLastError();
     delete b;
     return set_error(TEMPI_ERR_NOMEM, "batch table allocation failed");
   }
   b->d_items = reinterpret_cast<BatchItem *>(b->dev);
-  b->d_tile_begin = reinterpret_cast<uint32_t *>(b->dev + tb_off);
-  b->d_tile_item = reinterpret_cast<uint32_t *>(b->dev + ti_off);
+  b->d_tiles = reinterpret_cast<Tile *>(b->dev + tl_off);
+  b->d_xcd_begin = reinterpret_cast<uint32_t *>(b->dev + xb_off);
   hipStream_t s = library_stream();
   std::vector<char> host(total, 0);
   if (!items.empty()) {
     memcpy(host.data(), items.data(), items_bytes);
-    memcpy(host.data() + tb_off, tile_begin.data(), tile_begin.size() * 4);
-    memcpy(host.data() + ti_off, tile_item.data(), tile_item.size() * 4);
+    memcpy(host.data() + tl_off, tile_list.data(),
+           tile_list.size() * sizeof(Tile));
   }
+  memcpy(host.data() + xb_off, xcd_begin.data(), xcd_begin.size() * 4);
   // one upload, outside any capture
   if (hipSuccess != hipMemcpyAsync(b->dev, host.data(), total,
                                    hipMemcpyHostToDevice, s) ||
@@ -274,8 +298,8 @@ TEMPI_EXPORT int tempi_batch_graph_launch(tempi_batch *b, void *stream) {
     TEMPI_HIP_CHECK(hipStreamBeginCapture(b->capture_stream,
                                           hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0,
-                       b->capture_stream, b->d_items, b->d_tile_begin,
-                       b->d_tile_item, b->ntiles, b->xcd_split);
+                       b->capture_stream, b->d_items, b->d_tiles,
+                       b->d_xcd_begin, b->ntiles, b->xcd_split);
     TEMPI_HIP_CHECK(hipStreamEndCapture(b->capture_stream, &b->graph));
     TEMPI_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
