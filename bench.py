@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="warmup + timed headline steps only (for rocprofv3)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="with --profile-only: also run one plain 256 MiB contiguous copy "
+                         "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share of the pack path; --gpus {args.gpus} is not "
@@ -120,6 +123,14 @@ def main():
     log(f"batched: {ms_per_step * 1e3:.2f} us/step, {value:.1f} GB/s packed, "
         f"batch kernel {batch_avg_us:.2f} us/launch (device)")
     if args.profile_only:
+        if args.calibrate_copy:
+            # 256 MiB read + 256 MiB written by the contiguous-copy kernel
+            cal = 256 << 20
+            a, b = ta.DeviceBuffer(cal), ta.DeviceBuffer(cal)
+            a.fill(1)
+            sb1 = ta.StridedBlock.make(0, cal, [cal], [1])
+            ta.pack_sb(sb1, a.addr(), 1, b.addr(), cal, 0, 1, stream)
+            ta.synchronize(stream)
         print(json.dumps({"profile_only": True, "ms_per_step": ms_per_step,
                           "batch_avg_us": batch_avg_us}))
         return
@@ -210,6 +221,7 @@ def main():
     extras["mpi_call_wall_us"] = call_us
 
     if not args.no_extras:
+        extras["large_blocks_cold"] = bench_large_blocks(ta, stream)
         extras["halo_batch"] = bench_halos(ta, m, tc, stream)
         extras["random_requests"] = bench_random_requests(ta, stream)
         extras["vector_2d"] = bench_vector(ta, m, tc, stream)
@@ -269,6 +281,45 @@ def main():
     pack_batch.destroy()
     unpack_batch.destroy()
     m.finalize()
+
+
+def bench_large_blocks(ta, stream):
+    """2D packs and unpacks of 256 MiB with block lengths 64 B - 4 KiB (stride
+    = 2 x block + 64 B), each timed as one launch after a read-only cache
+    flush, with the variant the autotuner picks."""
+    import numpy as np
+
+    n = 256 << 20
+    src = ta.DeviceBuffer(3 << 30)
+    src.fill(3)
+    dst = ta.DeviceBuffer(n)
+    dst.fill(4)
+    res = {}
+    ev = [ta.Event(), ta.Event()]
+    for bl in (64, 256, 1024, 4096):
+        rows = n // bl
+        stride = 2 * bl + 64
+        sb = ta.StridedBlock.make(0, rows * stride, [bl, rows], [1, stride])
+        for d, name in ((ta.DIR_PACK, "pack"), (ta.DIR_UNPACK, "unpack")):
+            v = ta.autotune(sb, 1, src.addr(), dst.addr(), d)[0]
+            ts = []
+            for _ in range(5):
+                ta.flush_caches()
+                ev[0].record(stream)
+                if d == ta.DIR_PACK:
+                    ta.pack_sb(sb, src.addr(), 1, dst.addr(), n, 0, v, stream)
+                else:
+                    ta.unpack_sb(sb, dst.addr(), n, src.addr(), 1, 0, v, stream)
+                ev[1].record(stream)
+                ta.synchronize(stream)
+                ts.append(ev[0].elapsed_ms(ev[1]) * 1e3)
+            us = float(np.median(ts))
+            gbs = 2 * n / us / 1e3
+            res[f"{name}_bl{bl}"] = {"variant": v, "us": round(us, 2),
+                                     "hbm_gbs": round(gbs, 1),
+                                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    log(f"large blocks (cold): {res}")
+    return res
 
 
 def bench_halos(ta, m, tc, stream):

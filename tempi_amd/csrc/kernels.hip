@@ -158,7 +158,8 @@ int heuristic_variant(const Geom &g, const char *strided, const char *packed,
   const int64_t L = g.L;
   const int64_t bytes = L * g.rows;
   if (L >= 1024) {
-    return bytes >= (8ll << 20) ? 1 : 0;
+    // large copies stream past the caches: nontemporal stores
+    return bytes >= (8ll << 20) ? 2 : 0;
   }
   if (L >= 512) {
     return 3;

@@ -145,7 +145,7 @@ def time_cases(cases, src, dst, log_every=2000, tag=""):
     for i, (sb, base, direction, ppos) in enumerate(cases):
         strided, packed = src.addr(base), dst.addr(ppos)
         size = sb.size()
-        reps = 5 if size < (1 << 20) else (3 if size < (8 << 20) else 2)
+        reps = 10 if size < (1 << 20) else (4 if size < (8 << 20) else 2)
         ta.check(L.tempi_mlp_features(ctypes.byref(sb), 1, strided, packed, direction, f))
         X[i] = np.frombuffer(f, np.float32)
         for v in range(nv):
@@ -195,10 +195,14 @@ def main(argv=None):
         dst.fill(2)
         t0 = time.time()
         tr = time_cases(generate(args.train, args.seed, args.max_packed), src, dst, tag="train")
-        ho = time_cases(generate(args.heldout, args.heldout_seed, args.max_packed), src, dst,
-                        tag="heldout")
+        hcases = generate(args.heldout, args.heldout_seed, args.max_packed)
+        ho = time_cases(hcases, src, dst, tag="heldout")
+        # a second, independent timing of the held-out copies: every choice
+        # (model, closed-form rule, and the autotuner's own pick from the
+        # first timing) is judged on it
+        ho2 = time_cases(hcases, src, dst, tag="heldout-retime")
         data = {"Xtr": tr[0], "Ttr": tr[1], "Htr": tr[2], "Btr": tr[3],
-                "Xho": ho[0], "Tho": ho[1], "Hho": ho[2], "Bho": ho[3]}
+                "Xho": ho[0], "Tho": ho[1], "Hho": ho[2], "Bho": ho[3], "Tho2": ho2[1]}
         print(f"[selector_train] timing took {time.time() - t0:.0f} s", flush=True)
         if args.dataset:
             np.savez_compressed(args.dataset, **data)
@@ -211,8 +215,10 @@ def main(argv=None):
     inv = 1.0 / np.maximum(std, 1e-3)
     norm = np.concatenate([mean, inv]).astype(np.float32)
     Xn = ((Xtr - mean) * inv).astype(np.float32)
+    # log2 time relative to the best variant, capped at 2x slower: the
+    # selector needs the ordering near the best, not how bad the worst is
     lt = np.log2(Ttr)
-    target = (lt - lt.mean(1, keepdims=True)).astype(np.float32)
+    target = np.minimum(lt - lt.min(1, keepdims=True), 1.0).astype(np.float32)
 
     rng = np.random.default_rng(args.seed)
     parts = []
@@ -253,18 +259,38 @@ def main(argv=None):
         pm = predict(X)
         best = T.argmin(1)
         idx = np.arange(len(T))
+        tb = T[idx, best]
         report[split] = {
             "model_regret": regret_stats(T, pm),
             "heuristic_regret": regret_stats(T, H),
             "model_picks_best": float((pm == best).mean()),
+            # regret weighted by time: sum of chosen times / sum of best times - 1
+            "model_regret_time_weighted": float(T[idx, pm].sum() / tb.sum() - 1),
+            "heuristic_regret_time_weighted": float(T[idx, H].sum() / tb.sum() - 1),
             # packed GB/s summed over the set: bytes / time of the chosen variant
             "gbs_model": float(B.sum() / (T[idx, pm].sum() * 1e-6) / 1e9),
             "gbs_heuristic": float(B.sum() / (T[idx, H].sum() * 1e-6) / 1e9),
-            "gbs_autotuner_best": float(B.sum() / (T[idx, best].sum() * 1e-6) / 1e9),
+            "gbs_autotuner_best": float(B.sum() / (tb.sum() * 1e-6) / 1e9),
         }
+        if split == "heldout" and "Tho2" in data:
+            T2 = data["Tho2"]
+            t2b = T2.min(1)
+            report["heldout_retimed"] = {
+                "model_regret": regret_stats(T2, pm),
+                "heuristic_regret": regret_stats(T2, H),
+                # the exhaustive autotuner's pick from the first timing,
+                # judged on the second: the noise floor of this measure
+                "autotuner_regret": regret_stats(T2, best),
+                "model_regret_time_weighted": float(T2[idx, pm].sum() / t2b.sum() - 1),
+                "autotuner_regret_time_weighted": float(T2[idx, best].sum() / t2b.sum() - 1),
+                "heuristic_regret_time_weighted": float(T2[idx, H].sum() / t2b.sum() - 1),
+                "gbs_model": float(B.sum() / (T2[idx, pm].sum() * 1e-6) / 1e9),
+                "gbs_heuristic": float(B.sum() / (T2[idx, H].sum() * 1e-6) / 1e9),
+                "gbs_autotuner": float(B.sum() / (T2[idx, best].sum() * 1e-6) / 1e9),
+            }
     with open(args.report, "w") as fh:
         json.dump(report, fh, indent=1)
-    print(json.dumps(report["heldout"]), flush=True)
+    print(json.dumps(report.get("heldout_retimed", report["heldout"])), flush=True)
     return 0
 
 
