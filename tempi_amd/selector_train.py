@@ -10,11 +10,15 @@
 2. Time every kernel variant on each with the autotuner's timer
    (tempi_time_variant: HIP events around back-to-back launches).
 3. Train the MLP (tempi_mlp_train: the library's HIP forward / backward /
-   Adam kernels) to predict each variant's log2 time relative to the mean
-   over variants; features are normalized by the training set's mean / std.
-4. On the held-out copies, report the regret of the model's choice, of the
-   closed-form rule, against the exhaustive autotuner's best, and the packed
-   GB/s each choice achieves; write the model file the library loads.
+   Adam kernels) to predict each variant's log2 time relative to the best
+   variant, capped at 2x; features are normalized by the training set's
+   mean / std.
+4. On the held-out copies, report the regret of the model's choice and of
+   the closed-form rule against the exhaustive autotuner's best, and the
+   packed GB/s each choice achieves -- on the timings the autotuner chose
+   from, and on an independent second timing that also judges the
+   autotuner's own pick (its regret there is the measurement's noise
+   floor); write the model file the library loads.
 """
 from __future__ import annotations
 
