@@ -113,6 +113,28 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+TEST_APP_SRC = os.path.join(ROOT, "tests", "c", "pack_unpack_app.cpp")
+TEST_APP = os.path.join(ROOT, "tests", "c", "pack_unpack_app")
+
+
+def build_test_app(verbose: bool = False) -> str:
+    """The native MPI test program (tests/c/pack_unpack_app.cpp), linked
+    against the MPI library only: the GPU test runs it with libtempi_amd.so
+    preloaded, the drop-in way."""
+    mpi = mpi_home()
+    rpath = [d for d in (system_libdir(), os.path.join(mpi, "lib")) if d]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", TEST_APP_SRC,
+           f"-I{os.path.join(mpi, 'include')}", f"-L{os.path.join(mpi, 'lib')}", "-lmpi",
+           f"-Wl,-rpath,{':'.join(rpath)}", "-Wl,--enable-new-dtags", "-o", TEST_APP]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"test app build failed: {' '.join(cmd)}\n{res.stderr}")
+    if verbose:
+        print(f"built {TEST_APP}")
+    return TEST_APP
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_test_app(verbose=True)
     sys.exit(0)
