@@ -150,3 +150,36 @@ def test_supported_extras(mpi):
     check_type(mpi, ("indexed_block", 2, [0, 5, 10, 15], tc.INT))
     check_type(mpi, ("hindexed", [4, 4, 4, 4], [0, 64, 1000, 1064], tc.BYTE))
     check_type(mpi, ("hvector", 4, 2, -24, ("contiguous", 3, tc.BYTE)))
+
+
+def test_equivalent_constructions_share_one_canonical_form(mpi):
+    """The reference's type_equivalence test (test/type_equivalence.cpp:51-61)
+    only compares sizes; here every construction of the same 3D layout must
+    lower to the same (start, counts, strides) and pack the same bytes."""
+    factories = (tc.make_byte_vn_hv_hv, tc.make_byte_v1_hv_hv, tc.make_byte_v_hv, tc.make_hi,
+                 tc.make_hib, tc.make_subarray, tc.make_subarray_v)
+    for cp, al in (((2, 3, 4), (10, 10, 10)), ((12, 5, 3), (16, 8, 4)), ((7, 1, 5), (9, 9, 9)),
+                   ((16, 16, 16), (16, 16, 16)), ((1, 4, 4), (3, 5, 7))):
+        forms = set()
+        packs = set()
+        for f in factories:
+            spec = f(cp, al)
+            h = mpi.type_from_spec(spec)
+            got = mpi.strided_block(h)
+            assert got is not None, (f.__name__, cp, al)
+            start, _, counts, strides = got[0].as_tuple()
+            forms.add((start, counts, strides))
+            o = orc.OracleType(spec)
+            buf = orc.splitmix_bytes(al[0] * al[1] * al[2] + 64, 3)
+            packs.add(o.pack(buf, 0, 1).tobytes())
+            mpi.free(h)
+        assert len(forms) == 1, (cp, al, forms)
+        assert len(packs) == 1
+
+
+def test_fortran_order_subarray(mpi):
+    # Fortran order: the first dimension is the fastest
+    sb = check_type(mpi, ("subarray", [6, 5, 4], [2, 3, 2], [1, 2, 1], "F", tc.INT))
+    assert list(sb.counts[: sb.ndims]) == [8, 3, 2]
+    assert list(sb.strides[: sb.ndims]) == [1, 24, 120]
+    assert sb.start == (1 + 2 * 6 + 1 * 30) * 4
