@@ -183,7 +183,8 @@ def main():
             kernels.append({"op": f"{kind}_{f['face']}", "us": round(us, 3),
                             "variant": ta.select_variant(
                                 f["sb"], 1, grid.addr(), f["packed"].addr(),
-                                ta.DIR_PACK if kind == "pack" else ta.DIR_UNPACK),
+                                ta.DIR_PACK if kind == "pack" else ta.DIR_UNPACK,
+                                ta.SELECT_DEFAULT),
                             "hbm_gbs": round(2 * f["size"] / us / 1e3, 1)})
     extras["single_kernels_warm"] = kernels
 
@@ -260,6 +261,7 @@ def main():
                     "unpacks them (library stream)",
         },
         "bit_exact": bit_exact,
+        "selector": "model" if ta.lib().tempi_mlp_loaded() else "closed-form rule",
         "roofline": {
             "bound": "hbm",
             "kernel": "k_batch",

@@ -131,7 +131,13 @@ def generate(n, seed, max_packed):
             sb, base, span = _random_case(rng, max_packed)
         direction = rng.randint(0, 1)
         ppos = rng.choice([0, 0, 0, 4, 8, rng.randint(0, 63)])
-        cases.append((sb, base, direction, ppos))
+        # several objects per call for a fifth of the copies, when they fit
+        count = 1
+        if rng.random() < 0.2:
+            c = rng.choice([2, 3, 4, 8])
+            if sb.extent * c + base < SRC_BYTES and sb.size() * c <= DST_BYTES:
+                count = c
+        cases.append((sb, base, direction, ppos, count))
     return cases
 
 
@@ -146,17 +152,17 @@ def time_cases(cases, src, dst, log_every=2000, tag=""):
     f = (ctypes.c_float * nf)()
     us = ctypes.c_float()
     t0 = time.time()
-    for i, (sb, base, direction, ppos) in enumerate(cases):
+    for i, (sb, base, direction, ppos, count) in enumerate(cases):
         strided, packed = src.addr(base), dst.addr(ppos)
-        size = sb.size()
+        size = sb.size() * count
         reps = 10 if size < (1 << 20) else (4 if size < (8 << 20) else 2)
-        ta.check(L.tempi_mlp_features(ctypes.byref(sb), 1, strided, packed, direction, f))
+        ta.check(L.tempi_mlp_features(ctypes.byref(sb), count, strided, packed, direction, f))
         X[i] = np.frombuffer(f, np.float32)
         for v in range(nv):
-            ta.check(L.tempi_time_variant(ctypes.byref(sb), 1, strided, packed, direction, v, 0,
-                                          reps, ctypes.byref(us)), "time_variant")
+            ta.check(L.tempi_time_variant(ctypes.byref(sb), count, strided, packed, direction, v,
+                                          0, reps, ctypes.byref(us)), "time_variant")
             T[i, v] = max(float(us.value), 1e-3)
-        heur[i] = ta.select_variant(sb, 1, strided, packed, direction, ta.SELECT_HEURISTIC)
+        heur[i] = ta.select_variant(sb, count, strided, packed, direction, ta.SELECT_HEURISTIC)
         nbytes[i] = size
         if log_every and (i + 1) % log_every == 0:
             print(f"[selector_train] {tag} timed {i + 1}/{len(cases)} in {time.time() - t0:.0f} s",
@@ -216,7 +222,9 @@ def main(argv=None):
     Xtr, Ttr = data["Xtr"], data["Ttr"]
     mean = Xtr.mean(0)
     std = Xtr.std(0)
-    inv = 1.0 / np.maximum(std, 1e-3)
+    # a floor on the scale: a feature that barely varies in training must not
+    # be amplified into a huge input when it does vary later
+    inv = 1.0 / np.maximum(std, 0.5)
     norm = np.concatenate([mean, inv]).astype(np.float32)
     Xn = ((Xtr - mean) * inv).astype(np.float32)
     # log2 time relative to the best variant, capped at 2x slower: the
