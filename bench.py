@@ -226,6 +226,7 @@ def main():
         extras["halo_batch"] = bench_halos(ta, m, tc, stream)
         extras["random_requests"] = bench_random_requests(ta, stream)
         extras["vector_2d"] = bench_vector(ta, m, tc, stream)
+        extras["system_tables"] = bench_system_tables(ta)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_grid_faces_518.json")
@@ -475,6 +476,36 @@ def bench_vector(ta, m, tc, stream):
     return {"mpi_pack_wall_us": round(float(np.median(ts)) * 1e6, 2),
             "kernel_us": round(dev * 1e6 / 200, 3),
             "packed_bytes": size}
+
+
+def bench_system_tables(ta):
+    """Measure the method model's tables (kernel launch, d2h/h2d curves, the
+    four 9x9 pack/unpack tables) and report a few of their points: 4 MiB in
+    256-B blocks packed to HBM (DEVICE) and to mapped host memory (ONE_SHOT,
+    bounded by the host link, not HBM)."""
+    from tempi_amd import system as tsys
+
+    tsys.clear()
+    t0 = time.perf_counter()
+    tsys.measure_performance(tsys.MEASURE_ALL, 0.01)
+    el = time.perf_counter() - t0
+    dev = tsys.get_table(tsys.PACK_DEVICE)
+    host = tsys.get_table(tsys.PACK_HOST)
+    uhost = tsys.get_table(tsys.UNPACK_HOST)
+    d2h = tsys.get_curve(tsys.D2H)
+    mib4 = 1 << 22
+    res = {
+        "measure_seconds": round(el, 2),
+        "kernel_launch_us": round(tsys.kernel_launch() * 1e6, 3),
+        "pack_device_4MiB_bl256_us": round(dev[8][8] * 1e6, 2),
+        "pack_device_64B_bl1_us": round(dev[0][0] * 1e6, 2),
+        "oneshot_pack_4MiB_bl256_gbs": round(mib4 / host[8][8] / 1e9, 2),
+        "oneshot_unpack_4MiB_bl256_gbs": round(mib4 / uhost[8][8] / 1e9, 2),
+        "d2h_8MiB_gbs": round((1 << 23) / d2h[23] / 1e9, 2),
+    }
+    tsys.clear()
+    log(f"system tables: {res}")
+    return res
 
 
 def cpu_baselines(args, m, orc, faces, host, np):

@@ -281,6 +281,79 @@ typedef struct tempi_counters {
 int tempi_counters_get(tempi_counters *out);
 int tempi_counters_reset(void);
 
+/* ---- measured system performance and the send-method model ----------------
+   The reference's system model (include/measure_system.hpp:26-106,
+   struct Performance): contiguous transfer curves (entry i = time of 2^i
+   bytes) and 9x9 pack/unpack tables (entry [i][j] = time of 2^(2i+6) bytes
+   in blocks of 2^j bytes at a 512-B stride), in seconds, stored as
+   perf.json in the cache directory with the reference's keys
+   (src/internal/measure_system.cpp:36-63).  Loaded at MPI_Init
+   (src/internal/measure_system.cu:32). */
+#define TEMPI_CURVE_D2H 0           /* "d2h"                      */
+#define TEMPI_CURVE_H2D 1           /* "h2d"                      */
+#define TEMPI_CURVE_INTRA_CPU_CPU 2 /* "intraNodeCpuCpuPingpong"  */
+#define TEMPI_CURVE_INTRA_GPU_GPU 3 /* "intraNodeGpuGpuPingpong"  */
+#define TEMPI_CURVE_INTER_CPU_CPU 4 /* "interNodeCpuCpuPingpong"  */
+#define TEMPI_CURVE_INTER_GPU_GPU 5 /* "interNodeGpuGpuPingpong"  */
+#define TEMPI_NUM_CURVES 6
+#define TEMPI_TABLE_PACK_DEVICE 0   /* "packDevice"   */
+#define TEMPI_TABLE_UNPACK_DEVICE 1 /* "unpackDevice" */
+#define TEMPI_TABLE_PACK_HOST 2     /* "packHost"     */
+#define TEMPI_TABLE_UNPACK_HOST 3   /* "unpackHost"   */
+#define TEMPI_NUM_TABLES 4
+#define TEMPI_PERF_TABLE_DIM 9      /* rows (bytes) and columns (block) */
+#define TEMPI_PERF_CURVE_LEN 24     /* 1 B .. 8 MiB */
+/* send methods for non-contiguous data
+   (reference: Performance::SendNonContigNd::Method, measure_system.hpp:92) */
+#define TEMPI_METHOD_UNKNOWN 0
+#define TEMPI_METHOD_DEVICE 1  /* pack in device memory, send from there   */
+#define TEMPI_METHOD_ONESHOT 2 /* pack straight into mapped host memory    */
+#define TEMPI_METHOD_STAGED 3  /* pack in device memory, copy to the host  */
+
+/* Interpolated time of `bytes` from a curve of n points (entry i = 2^i B),
+   +inf when n == 0.  reference: interp_time / interp_time_opt,
+   src/internal/measure_system.cpp:182-214. */
+double tempi_interp_time(const double *times, int n, int64_t bytes);
+/* Interpolated time from a rows x cols table (row-major; row i = 2^(2i+6)
+   bytes, column j = block length 2^j), +inf when rows == 0.
+   reference: interp_2d / interp_2d_opt, measure_system.cpp:216-297. */
+double tempi_interp_2d(const double *table, int rows, int cols, int64_t bytes,
+                       int64_t block_length);
+
+/* The process-wide tables.  get: *n (or *rows, *cols) receives the stored
+   size; up to `cap` values are copied.  set: n == 0 clears the entry. */
+int tempi_perf_clear(void);
+int tempi_perf_get_curve(int curve, double *times, int cap, int *n);
+int tempi_perf_set_curve(int curve, const double *times, int n);
+int tempi_perf_get_table(int table, double *times, int cap, int *rows,
+                         int *cols);
+int tempi_perf_set_table(int table, const double *times, int rows, int cols);
+double tempi_perf_kernel_launch(void); /* "cudaKernelLaunch", seconds */
+int tempi_perf_set_kernel_launch(double seconds);
+/* perf.json import / export; NULL = <cache dir>/perf.json.
+   reference: import_performance / export_performance,
+   measure_system.cpp:140-180. */
+int tempi_perf_load(const char *path);
+int tempi_perf_save(const char *path);
+/* Fill the empty entries this GPU can measure (reference: measure_performance,
+   src/internal/measure_system.cu:381-609): bit 0 kernel launch, bit 1 the
+   d2h/h2d curves, bit 2 the device pack/unpack tables, bit 3 the mapped-host
+   (ONE_SHOT) tables.  Ping-pong curves need two ranks and are only
+   imported.  Each point takes at least 7 samples and at most
+   `seconds_per_point` (<= 0: 0.05 s) or 500 samples; its time is their
+   trimean (src/internal/statistics.cpp:30-39). */
+int tempi_perf_measure(int what, double seconds_per_point);
+/* Modelled time of sending `bytes` of non-contiguous data in blocks of
+   `block_length` by `method`; TEMPI_ERR_UNSUPPORTED when an input is missing.
+   reference: DeviceND/OneshotND/StagedND::model, src/internal/sender.cpp:
+   128-137, 180-191, 238-248. */
+int tempi_perf_model(int method, int colocated, int64_t bytes,
+                     int64_t block_length, double *seconds);
+/* DEVICE or ONE_SHOT, whichever models faster (ties and unknowns: DEVICE);
+   cached per (colocated, bytes, block_length).  reference:
+   SendRecvND::send, src/internal/sender.cpp:251-273. */
+int tempi_choose_method(int colocated, int64_t bytes, int64_t block_length);
+
 #ifdef __cplusplus
 }
 #endif

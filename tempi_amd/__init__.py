@@ -193,6 +193,20 @@ def _prototypes(L):
         "tempi_mlp_save": (_I, [ctypes.c_char_p, _P, _P]),
         "tempi_mlp_load": (_I, [ctypes.c_char_p]),
         "tempi_mlp_loaded": (_I, []),
+        "tempi_interp_time": (ctypes.c_double, [_P, _I, _I64]),
+        "tempi_interp_2d": (ctypes.c_double, [_P, _I, _I, _I64, _I64]),
+        "tempi_perf_clear": (_I, []),
+        "tempi_perf_get_curve": (_I, [_I, _P, _I, ctypes.POINTER(_I)]),
+        "tempi_perf_set_curve": (_I, [_I, _P, _I]),
+        "tempi_perf_get_table": (_I, [_I, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+        "tempi_perf_set_table": (_I, [_I, _P, _I, _I]),
+        "tempi_perf_kernel_launch": (ctypes.c_double, []),
+        "tempi_perf_set_kernel_launch": (_I, [ctypes.c_double]),
+        "tempi_perf_load": (_I, [ctypes.c_char_p]),
+        "tempi_perf_save": (_I, [ctypes.c_char_p]),
+        "tempi_perf_measure": (_I, [_I, ctypes.c_double]),
+        "tempi_perf_model": (_I, [_I, _I, _I64, _I64, ctypes.POINTER(ctypes.c_double)]),
+        "tempi_choose_method": (_I, [_I, _I64, _I64]),
     }
     for name, (res, args) in protos.items():
         fn = getattr(L, name)

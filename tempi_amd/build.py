@@ -29,6 +29,7 @@ SOURCES = [
     "autotune.hip",
     "batch.hip",
     "mlp.hip",
+    "perf.hip",
 ]
 
 ARCH = os.environ.get("TEMPI_ARCH", "gfx950")

@@ -42,6 +42,11 @@ Env &env() {
   return e;
 }
 
+namespace {
+std::once_flag g_env_once;
+}
+void ensure_env() { std::call_once(g_env_once, read_environment); }
+
 // reference: src/internal/env.cpp:23-107.  Only the knobs of the pack path
 // are read; the others belong to subsystems this library does not carry.
 void read_environment() {

@@ -55,6 +55,7 @@ struct Env {
 };
 Env &env();
 void read_environment();
+void ensure_env(); // read_environment() once per process
 
 // ---------------------------------------------------------------------------
 // counters (reference: include/counters.hpp:86-193)
@@ -195,6 +196,9 @@ bool model_available();
 int model_variant(const Geom &g, const char *strided, const char *packed,
                   int dir, const std::string &key); // -1: no model
 void model_precommit(const tempi_strided_block &sb);
+
+// system performance tables (perf.hip): load <cache dir>/perf.json if present
+void perf_init();
 
 // exhaustive search; fills best and, when given, the best time per variant
 int autotune(const Geom &g, const char *strided, char *packed, int dir,

@@ -41,9 +41,6 @@ bool lookup(MPI_Datatype t, Canon *out) {
   return true;
 }
 
-std::once_flag g_env_once;
-void ensure_env() { std::call_once(g_env_once, read_environment); }
-
 // Shared body of pack and unpack on device-addressable buffers.
 // `strided` / `packed` are device addresses of the strided buffer origin and
 // of the first packed byte.
@@ -86,6 +83,7 @@ TEMPI_EXPORT int MPI_Init(int *argc, char ***argv) {
   if (have_device()) {
     (void)ensure_runtime();
   }
+  perf_init();
   return err;
 }
 
@@ -99,6 +97,7 @@ TEMPI_EXPORT int MPI_Init_thread(int *argc, char ***argv, int required,
   if (have_device()) {
     (void)ensure_runtime();
   }
+  perf_init();
   return err;
 }
 

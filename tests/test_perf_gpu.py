@@ -1,0 +1,87 @@
+"""The system-performance tables measured on the GPU with the library's own
+pack kernels (reference: measure_performance,
+src/internal/measure_system.cu:381-609), and the method choice on them."""
+import math
+
+import numpy as np
+import pytest
+
+import tempi_amd as ta
+from oracle import perf_ref
+from tempi_amd import system as tsys
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def measured():
+    assert ta.device_count() > 0
+    tsys.clear()
+    print("measuring system tables (5 ms per point)", flush=True)
+    tsys.measure_performance(tsys.MEASURE_ALL, 0.005)
+    snap = {
+        "launch": tsys.kernel_launch(),
+        "curves": [tsys.get_curve(c) for c in range(6)],
+        "tables": [tsys.get_table(t) for t in range(4)],
+    }
+    yield snap
+    tsys.clear()
+
+
+def test_tables_are_complete_and_positive(measured):
+    assert 0 < measured["launch"] < 1e-3
+    for c in (tsys.D2H, tsys.H2D):
+        v = measured["curves"][c]
+        assert v.shape == (tsys.CURVE_LEN,) and np.isfinite(v).all() and (v > 0).all()
+    for c in (tsys.INTRA_CPU_CPU, tsys.INTRA_GPU_GPU, tsys.INTER_CPU_CPU, tsys.INTER_GPU_GPU):
+        assert measured["curves"][c].size == 0  # two ranks needed: not measured
+    for t in range(4):
+        v = measured["tables"][t]
+        assert v.shape == (9, 9) and np.isfinite(v).all() and (v > 0).all(), t
+        assert (v < 1.0).all(), t
+
+
+def test_tables_have_the_expected_shape_of_costs(measured):
+    dev = measured["tables"][tsys.PACK_DEVICE]
+    host = measured["tables"][tsys.PACK_HOST]
+    # 4 MiB in 1-B blocks costs more than 64 B in 1-B blocks
+    assert dev[8][0] > dev[0][0] and host[8][0] > host[0][0]
+    # at 4 MiB, 256-B blocks move faster than 1-B blocks (fewer rows, whole lines)
+    assert dev[8][8] < dev[8][0]
+    # 8 MiB host copies take longer than 1-B ones
+    assert measured["curves"][tsys.D2H][23] > measured["curves"][tsys.D2H][0]
+
+
+def test_measure_fills_only_empty_entries(measured):
+    tsys.set_table(tsys.PACK_DEVICE, np.full((9, 9), 7.0))
+    tsys.measure_performance(tsys.MEASURE_DEVICE_TABLES, 0.002)
+    assert (tsys.get_table(tsys.PACK_DEVICE) == 7.0).all()
+    # restore the measured table for the tests that follow
+    tsys.set_table(tsys.PACK_DEVICE, measured["tables"][tsys.PACK_DEVICE])
+
+
+def test_choice_on_measured_tables_is_the_argmin(measured, tmp_path):
+    # ping-pong curves cannot be measured on one GPU: give both links the
+    # same synthetic cost so the choice is decided by the measured tables
+    send = [5e-6 + 2 ** i / 25e9 for i in range(24)]
+    tsys.set_curve(tsys.INTRA_CPU_CPU, send)
+    tsys.set_curve(tsys.INTRA_GPU_GPU, send)
+    path = str(tmp_path / "perf.json")
+    tsys.export_performance(path)
+    perf = {}
+    for c, k in enumerate(tsys.CURVE_KEYS):
+        perf[k] = tsys.get_curve(c).tolist()
+    for t, k in enumerate(tsys.TABLE_KEYS):
+        perf[k] = tsys.get_table(t).tolist()
+    for nbytes in (64, 1000, 1 << 16, 1 << 20, 1 << 22):
+        for bl in (1, 8, 64, 256):
+            o = tsys.model_oneshot(True, nbytes, bl)
+            d = tsys.model_device(True, nbytes, bl)
+            assert o == pytest.approx(perf_ref.model(perf, "oneshot", True, nbytes, bl), rel=1e-12)
+            assert d == pytest.approx(perf_ref.model(perf, "device", True, nbytes, bl), rel=1e-12)
+            want = tsys.METHOD_ONESHOT if o < d else tsys.METHOD_DEVICE
+            assert tsys.choose_method(True, nbytes, bl) == want
+    tsys.clear()
+    tsys.import_performance(path)
+    assert (tsys.get_table(tsys.UNPACK_HOST) == np.array(perf["unpackHost"])).all()
+    assert not math.isinf(tsys.model_staged(True, 1 << 20, 64))
