@@ -37,6 +37,9 @@ struct BatchItem {
 
 constexpr uint32_t kXcds = 8;
 
+// requests smaller than this use the closed-form variant rule in a batch
+constexpr int64_t kBatchSelectMinBytes = 1ll << 20;
+
 // one block-pass over units [u0, u0 + GROUPS*K) of a request
 template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
 __device__ __forceinline__ void copy_tile(const KParams &p, uint32_t u0) {
@@ -185,8 +188,15 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
     const int dir = r.direction == TEMPI_DIR_UNPACK ? 1 : 0;
     int variant = r.variant;
     if (variant < 0 || variant >= kNumVariants) {
+      // The autotuner and the learned selector rank variants by the time of
+      // one launch of the copy alone.  Below about a megabyte that time is
+      // launch latency, which says nothing about what the copy's tiles cost
+      // inside a shared launch; such requests take the closed-form rule,
+      // which sizes tiles for throughput.
+      const int m = g.L * g.rows < kBatchSelectMinBytes ? TEMPI_SELECT_HEURISTIC
+                                                         : method;
       variant = choose(g, static_cast<const char *>(ds), static_cast<char *>(dp),
-                       dir, method, library_stream())
+                       dir, m, library_stream())
                     .variant;
     }
     b->variants.back() = variant;
