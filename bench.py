@@ -227,6 +227,7 @@ def main():
         extras["random_requests"] = bench_random_requests(ta, stream)
         extras["vector_2d"] = bench_vector(ta, m, tc, stream)
         extras["system_tables"] = bench_system_tables(ta)
+        extras["vector_sweep"] = bench_sweep(ta, m, tc, stream)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_grid_faces_518.json")
@@ -288,39 +289,34 @@ def main():
 
 def bench_large_blocks(ta, stream):
     """2D packs and unpacks of 256 MiB with block lengths 64 B - 4 KiB (stride
-    = 2 x block + 64 B), each timed as one launch after a read-only cache
-    flush, with the variant the autotuner picks."""
-    import numpy as np
-
+    = 2 x block + 64 B), each launch timed after a read-only cache sweep
+    (the library times copies this large that way), with the library's
+    default choice (what MPI_Pack runs) and with the autotuner's (variant,
+    grid cap)."""
     n = 256 << 20
     src = ta.DeviceBuffer(3 << 30)
     src.fill(3)
     dst = ta.DeviceBuffer(n)
     dst.fill(4)
     res = {}
-    ev = [ta.Event(), ta.Event()]
     for bl in (64, 256, 1024, 4096):
         rows = n // bl
         stride = 2 * bl + 64
         sb = ta.StridedBlock.make(0, rows * stride, [bl, rows], [1, stride])
         for d, name in ((ta.DIR_PACK, "pack"), (ta.DIR_UNPACK, "unpack")):
-            v = ta.autotune(sb, 1, src.addr(), dst.addr(), d)[0]
-            ts = []
-            for _ in range(5):
-                ta.flush_caches()
-                ev[0].record(stream)
-                if d == ta.DIR_PACK:
-                    ta.pack_sb(sb, src.addr(), 1, dst.addr(), n, 0, v, stream)
-                else:
-                    ta.unpack_sb(sb, dst.addr(), n, src.addr(), 1, 0, v, stream)
-                ev[1].record(stream)
-                ta.synchronize(stream)
-                ts.append(ev[0].elapsed_ms(ev[1]) * 1e3)
-            us = float(np.median(ts))
-            gbs = 2 * n / us / 1e3
-            res[f"{name}_bl{bl}"] = {"variant": v, "us": round(us, 2),
-                                     "hbm_gbs": round(gbs, 1),
-                                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            vd = ta.select_variant(sb, 1, src.addr(), dst.addr(), d, ta.SELECT_DEFAULT)
+            us_d = ta.time_variant(sb, 1, src.addr(), dst.addr(), d, vd, 0, 5)
+            va, ca, _ = ta.autotune(sb, 1, src.addr(), dst.addr(), d)
+            us_a = ta.time_variant(sb, 1, src.addr(), dst.addr(), d, va, ca, 5)
+            gd, ga = 2 * n / us_d / 1e3, 2 * n / us_a / 1e3
+            res[f"{name}_bl{bl}"] = {
+                "default": {"variant": vd, "us": round(us_d, 2), "hbm_gbs": round(gd, 1),
+                            "frac": round(gd / HBM_PEAK_GBS, 4)},
+                "autotuned": {"variant": va, "grid_cap": ca, "us": round(us_a, 2),
+                              "hbm_gbs": round(ga, 1), "frac": round(ga / HBM_PEAK_GBS, 4)},
+            }
+    src.free()
+    dst.free()
     log(f"large blocks (cold): {res}")
     return res
 
@@ -476,6 +472,68 @@ def bench_vector(ta, m, tc, stream):
     return {"mpi_pack_wall_us": round(float(np.median(ts)) * 1e6, 2),
             "kernel_us": round(dev * 1e6 / 200, 3),
             "packed_bytes": size}
+
+
+def bench_sweep(ta, m, tc, stream, ncases=48, max_span=8 << 30):
+    """The 2D vector / 3D hvector-of-vector byte sweep (block lengths 1 B -
+    1 KiB, stride = block + U{0..4096}, base offset U{0..15}, 1 KiB - 256 MiB
+    packed; case k uses seed 1000 + k), committed through the interposer and
+    packed and unpacked with the library's default choice.  Cases whose
+    strided span exceeds `max_span` are skipped and counted.  Each copy is
+    timed warm (one untimed launch, then the mean of 3 between HIP events);
+    copies this large are HBM-resident anyway."""
+    import random
+
+    import numpy as np
+
+    cases, skipped = [], 0
+    ev = [ta.Event(), ta.Event()]
+    for k in range(ncases):
+        spec, base, packed_bytes = tc.sweep_type(random.Random(1000 + k))
+        h = m.type_from_spec(spec)
+        sb, size = m.strided_block(h)
+        m.free(h)
+        span = base + sb.extent
+        if span > max_span:
+            skipped += 1
+            continue
+        src = ta.DeviceBuffer(span + 64)
+        dst = ta.DeviceBuffer(size + 64)
+        row = {"block": int(sb.counts[0]) if sb.ndims else size, "packed_bytes": size}
+        for d, name in ((ta.DIR_PACK, "pack"), (ta.DIR_UNPACK, "unpack")):
+            def once():
+                if d == ta.DIR_PACK:
+                    ta.pack_sb(sb, src.addr(base), 1, dst.addr(), size, 0, -1, stream)
+                else:
+                    ta.unpack_sb(sb, dst.addr(), size, src.addr(base), 1, 0, -1, stream)
+            once()
+            ev[0].record(stream)
+            for _ in range(3):
+                once()
+            ev[1].record(stream)
+            ta.synchronize(stream)
+            us = ev[0].elapsed_ms(ev[1]) * 1e3 / 3
+            row[f"{name}_us"] = round(us, 2)
+            row[f"{name}_frac"] = round(2 * size / us / 1e3 / HBM_PEAK_GBS, 4)
+            row[f"{name}_variant"] = ta.select_variant(sb, 1, src.addr(base), dst.addr(), d,
+                                                       ta.SELECT_DEFAULT)
+        src.free()
+        dst.free()
+        cases.append(row)
+    res = {"cases": len(cases), "skipped_span_over_8GiB": skipped}
+    for name in ("pack", "unpack"):
+        tot_b = sum(c["packed_bytes"] for c in cases)
+        tot_t = sum(c[f"{name}_us"] for c in cases)
+        res[f"{name}_packed_gbs_all"] = round(tot_b / tot_t / 1e3, 1)
+        big = [c for c in cases if c["block"] >= 64 and c["packed_bytes"] >= (16 << 20)]
+        if big:
+            fr = [c[f"{name}_frac"] for c in big]
+            res[f"{name}_hbm_frac_block_ge64_ge16MiB"] = {
+                "n": len(big), "median": round(float(np.median(fr)), 4),
+                "min": round(float(np.min(fr)), 4)}
+    res["per_case"] = cases
+    log(f"vector sweep: { {k: v for k, v in res.items() if k != 'per_case'} }")
+    return res
 
 
 def bench_system_tables(ta):

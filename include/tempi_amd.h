@@ -115,7 +115,8 @@ typedef struct tempi_variant_info {
   int32_t lanes_per_row; /* lanes of a wavefront that share one row   */
   int32_t word_bytes;    /* bytes per load/store                      */
   int32_t unroll;        /* words each lane moves per work unit       */
-  int32_t nontemporal;   /* packed-side stores bypass caches          */
+  int32_t nontemporal_stores; /* stores marked nontemporal           */
+  int32_t nontemporal_loads;  /* loads marked nontemporal            */
 } tempi_variant_info;
 int tempi_num_variants(void);
 int tempi_variant_info_get(int variant, tempi_variant_info *info);
@@ -151,8 +152,10 @@ int tempi_unpack_sb(const tempi_strided_block *sb, const void *inbuf,
    methods offline (src/internal/measure_system.cu:514-609).
    tempi_time_variant: average device time (us) of one launch of `variant`
    with grid cap index `grid_cap` (0..tempi_num_grid_caps()-1) over `reps`
-   back-to-back launches on the library stream.  The packed (pack) or strided
-   (unpack) buffer is written.
+   back-to-back launches on the library stream; copies of 64 MiB or more
+   are timed one launch at a time, each after the caches are emptied (data
+   that large is not cache-resident between real calls).  The packed (pack)
+   or strided (unpack) buffer is written.
    tempi_autotune: time every variant x grid cap, return the fastest and,
    if per_variant_us is not NULL, the best time of each variant
    (tempi_num_variants() floats).  Setting TEMPI_SELECT=autotune makes

@@ -87,7 +87,8 @@ class VariantInfo(ctypes.Structure):
         ("lanes_per_row", ctypes.c_int32),
         ("word_bytes", ctypes.c_int32),
         ("unroll", ctypes.c_int32),
-        ("nontemporal", ctypes.c_int32),
+        ("nontemporal_stores", ctypes.c_int32),
+        ("nontemporal_loads", ctypes.c_int32),
     ]
 
 

@@ -21,7 +21,75 @@ namespace tempi {
 
 // multiples of the CU count; 0 = the launch code's default (16 per CU)
 const int kGridCaps[] = {0, 2, 4, 8, 32};
+
+// Copies of at least kColdBytes packed bytes are timed one launch at a time,
+// each after a read-only sweep of kColdFlushBytes (twice the 256 MB
+// Infinity Cache) empties the caches: data that large cannot stay cached
+// between real calls, and back-to-back repeats would rank the variants by
+// how they cope with the previous repeat's dirty lines instead.
+constexpr int64_t kColdBytes = 64ll << 20;
+constexpr int64_t kColdFlushBytes = 512ll << 20;
 const int kNumGridCaps = sizeof(kGridCaps) / sizeof(kGridCaps[0]);
+
+// Reads one dword per 64 B of `buf` and writes one word per workgroup: the
+// caches end up full of clean lines of `buf`, so a kernel timed after it
+// starts from HBM without write-backs of someone else's dirty data.
+__global__ void __launch_bounds__(256) k_touch(const uint32_t *__restrict__ buf,
+                                               uint64_t words, uint32_t *sink) {
+  uint32_t acc = 0;
+  for (uint64_t i = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 16; i < words;
+       i += (uint64_t)gridDim.x * 256 * 16) {
+    acc ^= buf[i];
+  }
+  __shared__ uint32_t s;
+  if (threadIdx.x == 0) {
+    s = 0;
+  }
+  __syncthreads();
+  atomicXor(&s, acc);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sink[blockIdx.x] = s;
+  }
+}
+
+struct FlushBuf {
+  std::mutex mu;
+  char *buf = nullptr;
+  int64_t bytes = 0;
+};
+FlushBuf &flush_buf() {
+  static FlushBuf f;
+  return f;
+}
+
+// Sweep `bytes` (<= 0: 1 GiB) of a library-owned buffer with k_touch on
+// `stream`; the buffer is allocated and filled on first use.
+int flush_caches_on(int64_t bytes, hipStream_t s) {
+  if (bytes <= 0) {
+    bytes = 1ll << 30;
+  }
+  FlushBuf &f = flush_buf();
+  std::lock_guard<std::mutex> lk(f.mu);
+  const int blocks = 2048;
+  if (f.bytes < bytes) {
+    if (f.buf) {
+      TEMPI_HIP_CHECK(hipStreamSynchronize(s));
+      TEMPI_HIP_CHECK(hipFree(f.buf));
+      f.buf = nullptr;
+      f.bytes = 0;
+    }
+    TEMPI_HIP_CHECK(hipMalloc(&f.buf, (size_t)bytes + blocks * 4));
+    TEMPI_HIP_CHECK(hipMemset(f.buf, 1, (size_t)bytes));
+    f.bytes = bytes;
+  }
+  hipLaunchKernelGGL(k_touch, dim3(blocks), dim3(256), 0, s,
+                     reinterpret_cast<const uint32_t *>(f.buf),
+                     (uint64_t)bytes / 4,
+                     reinterpret_cast<uint32_t *>(f.buf + f.bytes));
+  TEMPI_HIP_CHECK(hipGetLastError());
+  return TEMPI_SUCCESS;
+}
 
 namespace {
 std::mutex g_tune_mu;
@@ -40,28 +108,46 @@ std::string shape_key_of(const Geom &g, const char *strided,
   return os.str();
 }
 
+bool times_cold(const Geom &g) { return g.L * g.rows >= kColdBytes; }
+
 int time_config(const Geom &g, const char *strided, char *packed, int dir,
                 int variant, int max_blocks, int reps, hipStream_t stream,
                 float *us) {
+  const bool cold = times_cold(g);
   hipEvent_t a = nullptr, b = nullptr;
   TEMPI_HIP_CHECK(hipEventCreate(&a));
   TEMPI_HIP_CHECK(hipEventCreate(&b));
   int err = launch_copy(g, strided, packed, dir, variant, stream, max_blocks);
-  if (!err) {
+  float total_ms = 0;
+  if (cold) {
+    // one launch at a time, each after the caches are emptied
+    for (int r = 0; r < std::max(reps, 1) && !err; ++r) {
+      err = flush_caches_on(kColdFlushBytes, stream);
+      if (!err) {
+        (void)hipEventRecord(a, stream);
+        err = launch_copy(g, strided, packed, dir, variant, stream, max_blocks);
+        (void)hipEventRecord(b, stream);
+      }
+      float ms = 0;
+      if (!err && (hipSuccess != hipEventSynchronize(b) ||
+                   hipSuccess != hipEventElapsedTime(&ms, a, b))) {
+        err = set_error(TEMPI_ERR_HIP, "event timing failed");
+      }
+      total_ms += ms;
+    }
+  } else if (!err) {
     (void)hipEventRecord(a, stream);
     for (int r = 0; r < reps && !err; ++r) {
       err = launch_copy(g, strided, packed, dir, variant, stream, max_blocks);
     }
     (void)hipEventRecord(b, stream);
+    if (!err && (hipSuccess != hipEventSynchronize(b) ||
+                 hipSuccess != hipEventElapsedTime(&total_ms, a, b))) {
+      err = set_error(TEMPI_ERR_HIP, "event timing failed");
+    }
   }
   if (!err) {
-    float ms = 0;
-    if (hipSuccess != hipEventSynchronize(b) ||
-        hipSuccess != hipEventElapsedTime(&ms, a, b)) {
-      err = set_error(TEMPI_ERR_HIP, "event timing failed");
-    } else {
-      *us = ms * 1e3f / (float)std::max(reps, 1);
-    }
+    *us = total_ms * 1e3f / (float)std::max(reps, 1);
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
@@ -139,38 +225,6 @@ __global__ void k_empty(int *p) {
   if (p) {
     *p = threadIdx.x;
   }
-}
-
-// Reads one dword per 64 B of `buf` and writes one word per workgroup: the
-// caches end up full of clean lines of `buf`, so a kernel timed after it
-// starts from HBM without write-backs of someone else's dirty data.
-__global__ void __launch_bounds__(256) k_touch(const uint32_t *__restrict__ buf,
-                                               uint64_t words, uint32_t *sink) {
-  uint32_t acc = 0;
-  for (uint64_t i = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 16; i < words;
-       i += (uint64_t)gridDim.x * 256 * 16) {
-    acc ^= buf[i];
-  }
-  __shared__ uint32_t s;
-  if (threadIdx.x == 0) {
-    s = 0;
-  }
-  __syncthreads();
-  atomicXor(&s, acc);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    sink[blockIdx.x] = s;
-  }
-}
-
-struct FlushBuf {
-  std::mutex mu;
-  char *buf = nullptr;
-  int64_t bytes = 0;
-};
-FlushBuf &flush_buf() {
-  static FlushBuf f;
-  return f;
 }
 
 } // namespace tempi
@@ -254,28 +308,10 @@ TEMPI_EXPORT int tempi_flush_caches(int64_t bytes) {
   if (err) {
     return err;
   }
-  if (bytes <= 0) {
-    bytes = 1ll << 30;
-  }
-  FlushBuf &f = flush_buf();
-  std::lock_guard<std::mutex> lk(f.mu);
-  const int blocks = 2048;
-  if (f.bytes < bytes) {
-    if (f.buf) {
-      TEMPI_HIP_CHECK(hipFree(f.buf));
-      f.buf = nullptr;
-      f.bytes = 0;
-    }
-    TEMPI_HIP_CHECK(hipMalloc(&f.buf, (size_t)bytes + blocks * 4));
-    TEMPI_HIP_CHECK(hipMemset(f.buf, 1, (size_t)bytes));
-    f.bytes = bytes;
-  }
   hipStream_t s = library_stream();
-  hipLaunchKernelGGL(k_touch, dim3(blocks), dim3(256), 0, s,
-                     reinterpret_cast<const uint32_t *>(f.buf),
-                     (uint64_t)bytes / 4,
-                     reinterpret_cast<uint32_t *>(f.buf + f.bytes));
-  TEMPI_HIP_CHECK(hipGetLastError());
+  if ((err = flush_caches_on(bytes, s))) {
+    return err;
+  }
   TEMPI_HIP_CHECK(hipStreamSynchronize(s));
   return TEMPI_SUCCESS;
 }

@@ -41,28 +41,28 @@ constexpr uint32_t kXcds = 8;
 constexpr int64_t kBatchSelectMinBytes = 1ll << 20;
 
 // one block-pass over units [u0, u0 + GROUPS*K) of a request
-template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
+template <int P, int W, int U, bool ALIGNED, int DIR, bool NTS, bool NTL>
 __device__ __forceinline__ void copy_tile(const KParams &p, uint32_t u0) {
   constexpr uint32_t GROUPS = kBlock / P;
-  copy_units<P, W, U, ALIGNED, DIR, NT>(p, u0 + threadIdx.x / P, GROUPS,
-                                        p.units, threadIdx.x % P);
+  copy_units<P, W, U, ALIGNED, DIR, NTS, NTL>(p, u0 + threadIdx.x / P, GROUPS,
+                                              p.units, threadIdx.x % P);
 }
 
 __device__ __forceinline__ void run_tile(int kind, const KParams &p,
                                          uint32_t u0) {
   switch (kind) {
-#define TEMPI_BCASE(I, P, W, U, NT)                                            \
+#define TEMPI_BCASE(I, P, W, U, NTS, NTL)                                      \
   case 4 * I + 0:                                                              \
-    copy_tile<P, W, U, false, 0, NT>(p, u0);                              \
+    copy_tile<P, W, U, false, 0, NTS, NTL>(p, u0);                             \
     break;                                                                     \
   case 4 * I + 1:                                                              \
-    copy_tile<P, W, U, false, 1, NT>(p, u0);                              \
+    copy_tile<P, W, U, false, 1, NTS, NTL>(p, u0);                             \
     break;                                                                     \
   case 4 * I + 2:                                                              \
-    copy_tile<P, W, U, true, 0, NT>(p, u0);                               \
+    copy_tile<P, W, U, true, 0, NTS, NTL>(p, u0);                              \
     break;                                                                     \
   case 4 * I + 3:                                                              \
-    copy_tile<P, W, U, true, 1, NT>(p, u0);                               \
+    copy_tile<P, W, U, true, 1, NTS, NTL>(p, u0);                              \
     break;
     TEMPI_VARIANTS(TEMPI_BCASE)
 #undef TEMPI_BCASE

@@ -61,12 +61,17 @@ __host__ __device__ constexpr int word_align(int W) {
   return (W & (W - 1)) == 0 ? W : 4;
 }
 
-// load W bytes from any byte address
-template <int W>
+// load W bytes from any byte address; NT marks the load nontemporal (the
+// line is filled for this access and is first in line for eviction)
+template <int W, bool NT>
 __device__ __forceinline__ typename WordT<W>::T load_word(const char *p) {
   typedef typename WordT<W>::T T;
   typedef T Tu __attribute__((aligned(1)));
-  return *reinterpret_cast<const Tu *>(p);
+  if constexpr (NT) {
+    return __builtin_nontemporal_load(reinterpret_cast<const Tu *>(p));
+  } else {
+    return *reinterpret_cast<const Tu *>(p);
+  }
 }
 
 // store W bytes to an address aligned to word_align(W)
@@ -114,7 +119,7 @@ template <int W, int U> struct UnitIO {
 // Load phase of work unit `u` (wave-uniform when P == 64).  DIR 0 = pack
 // (strided -> packed), 1 = unpack (packed -> strided).  Head and tail bytes
 // of a row whose destination is not word-aligned are copied here directly.
-template <int P, int W, int U, bool ALIGNED, int DIR>
+template <int P, int W, int U, bool ALIGNED, int DIR, bool NTL>
 __device__ __forceinline__ void load_unit(const KParams &p, uint32_t u,
                                           uint32_t lane, UnitIO<W, U> &io) {
   constexpr int A = word_align(W);
@@ -152,7 +157,7 @@ __device__ __forceinline__ void load_unit(const KParams &p, uint32_t u,
   for (int k = 0; k < U; ++k) {
     const uint32_t w = io.w + k * P;
     if (w < io.n) {
-      io.v[k] = load_word<W>(s + (size_t)w * W);
+      io.v[k] = load_word<W, NTL>(s + (size_t)w * W);
     }
   }
 }
@@ -174,8 +179,9 @@ __host__ __device__ constexpr int units_in_flight(int U) {
 }
 
 // Move units first, first + stride, ... (K of them, those below `end`):
-// all loads, then all stores.
-template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
+// all loads, then all stores.  NTS / NTL: nontemporal stores / loads of the
+// body words.
+template <int P, int W, int U, bool ALIGNED, int DIR, bool NTS, bool NTL>
 __device__ __forceinline__ void copy_units(const KParams &p, uint32_t first,
                                            uint32_t stride, uint32_t end,
                                            uint32_t lane) {
@@ -190,34 +196,41 @@ __device__ __forceinline__ void copy_units(const KParams &p, uint32_t first,
       u = __builtin_amdgcn_readfirstlane(u);
     }
     if (u < end) {
-      load_unit<P, W, U, ALIGNED, DIR>(p, u, lane, io[k]);
+      load_unit<P, W, U, ALIGNED, DIR, NTL>(p, u, lane, io[k]);
     }
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (first + k * stride < end) {
-      store_unit<P, W, U, NT>(io[k]);
+      store_unit<P, W, U, NTS>(io[k]);
     }
   }
 }
 
 constexpr int kBlock = 256;
 
-// The variant list, as X(index, P, W, U, NT).
+// The variant list, as X(index, P, W, U, NTS, NTL): P lanes per row, W-byte
+// words, U words per lane per unit, nontemporal stores / loads.
 #define TEMPI_VARIANTS(X)                                                      \
-  X(0, 64, 16, 1, false) /* long rows, 1 KiB per wavefront unit        */     \
-  X(1, 64, 16, 4, false) /* long rows, 4 KiB per unit                  */     \
-  X(2, 64, 16, 4, true)  /* as 1 with nontemporal stores               */     \
-  X(3, 32, 16, 2, false) /* rows around 512 B - 1 KiB                  */     \
-  X(4, 16, 16, 2, false) /* rows around 256 - 512 B                    */     \
-  X(5, 8, 16, 2, false)  /* rows around 128 - 256 B                    */     \
-  X(6, 4, 16, 2, false)  /* rows around 64 - 128 B                     */     \
-  X(7, 1, 16, 4, false)  /* lane per row, 16-B words                   */     \
-  X(8, 1, 8, 4, false)   /* lane per row, 8-B words                    */     \
-  X(9, 1, 4, 4, false)   /* lane per row, 4-B words                    */     \
-  X(10, 16, 4, 4, false) /* 4-B words across 16 lanes                  */     \
-  X(11, 1, 1, 16, false) /* lane per row, bytes                        */     \
-  X(12, 1, 12, 2, false) /* lane per row, 12-B words (3-float rows)    */
+  X(0, 64, 16, 1, false, false) /* long rows, 1 KiB per wavefront unit */     \
+  X(1, 64, 16, 4, false, false) /* long rows, 4 KiB per unit           */     \
+  X(2, 64, 16, 4, true, false)  /* as 1, nontemporal stores            */     \
+  X(3, 32, 16, 2, false, false) /* rows around 512 B - 1 KiB           */     \
+  X(4, 16, 16, 2, false, false) /* rows around 256 - 512 B             */     \
+  X(5, 8, 16, 2, false, false)  /* rows around 128 - 256 B             */     \
+  X(6, 4, 16, 2, false, false)  /* rows around 64 - 128 B              */     \
+  X(7, 1, 16, 4, false, false)  /* lane per row, 16-B words            */     \
+  X(8, 1, 8, 4, false, false)   /* lane per row, 8-B words             */     \
+  X(9, 1, 4, 4, false, false)   /* lane per row, 4-B words             */     \
+  X(10, 16, 4, 4, false, false) /* 4-B words across 16 lanes           */     \
+  X(11, 1, 1, 16, false, false) /* lane per row, bytes                 */     \
+  X(12, 1, 12, 2, false, false) /* lane per row, 12-B words (3 floats) */     \
+  X(13, 64, 16, 4, false, true) /* as 1, nontemporal loads             */     \
+  X(14, 32, 16, 2, false, true) /* as 3, nontemporal loads             */     \
+  X(15, 16, 16, 2, false, true) /* as 4, nontemporal loads             */     \
+  X(16, 8, 16, 2, false, true)  /* as 5, nontemporal loads             */     \
+  X(17, 4, 16, 2, false, true)  /* as 6, nontemporal loads             */     \
+  X(18, 1, 8, 4, false, true)   /* as 8, nontemporal loads             */
 
 // Fill the kernel parameters of one copy with `variant`; `units` may not fit
 // 32 bits, which is reported as TEMPI_ERR_UNSUPPORTED.

@@ -146,7 +146,9 @@ struct Geom {
 int make_geom(const tempi_strided_block &sb, int64_t count, Geom *g);
 
 struct VariantDesc {
-  int P, W, U, NT;
+  int P, W, U;
+  int NTS; // nontemporal stores
+  int NTL; // nontemporal loads
 };
 extern const VariantDesc kVariants[];
 extern const int kNumVariants;

@@ -280,7 +280,8 @@ TEMPI_EXPORT int tempi_variant_info_get(int variant, tempi_variant_info *info) {
   info->lanes_per_row = kVariants[variant].P;
   info->word_bytes = kVariants[variant].W;
   info->unroll = kVariants[variant].U;
-  info->nontemporal = kVariants[variant].NT;
+  info->nontemporal_stores = kVariants[variant].NTS;
+  info->nontemporal_loads = kVariants[variant].NTL;
   return TEMPI_SUCCESS;
 }
 

@@ -41,7 +41,7 @@ FastDiv make_fastdiv(uint32_t d) {
   return f;
 }
 
-template <int P, int W, int U, bool ALIGNED, int DIR, bool NT>
+template <int P, int W, int U, bool ALIGNED, int DIR, bool NTS, bool NTL>
 __global__ void __launch_bounds__(kBlock) k_strided_copy(KParams p) {
   constexpr uint32_t GROUPS = kBlock / P;
   constexpr uint32_t K = units_in_flight(U);
@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kBlock) k_strided_copy(KParams p) {
   const uint32_t step = gridDim.x * GROUPS;
   for (uint32_t u = blockIdx.x * GROUPS + threadIdx.x / P; u < p.units;
        u += step * K) {
-    copy_units<P, W, U, ALIGNED, DIR, NT>(p, u, step, p.units, lane);
+    copy_units<P, W, U, ALIGNED, DIR, NTS, NTL>(p, u, step, p.units, lane);
   }
 }
 
@@ -57,18 +57,18 @@ __global__ void __launch_bounds__(kBlock) k_strided_copy(KParams p) {
 // variant table
 // ---------------------------------------------------------------------------
 
-#define TEMPI_VDESC(I, P, W, U, NT) {P, W, U, NT ? 1 : 0},
+#define TEMPI_VDESC(I, P, W, U, NTS, NTL) {P, W, U, NTS ? 1 : 0, NTL ? 1 : 0},
 const VariantDesc kVariants[] = {TEMPI_VARIANTS(TEMPI_VDESC)};
 #undef TEMPI_VDESC
 const int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 typedef void (*KernelFn)(KParams);
 
-#define TEMPI_KROW(I, P, W, U, NT)                                                \
-  {{k_strided_copy<P, W, U, false, 0, NT>,                                     \
-    k_strided_copy<P, W, U, false, 1, NT>},                                    \
-   {k_strided_copy<P, W, U, true, 0, NT>,                                      \
-    k_strided_copy<P, W, U, true, 1, NT>}},
+#define TEMPI_KROW(I, P, W, U, NTS, NTL)                                         \
+  {{k_strided_copy<P, W, U, false, 0, NTS, NTL>,                               \
+    k_strided_copy<P, W, U, false, 1, NTS, NTL>},                              \
+   {k_strided_copy<P, W, U, true, 0, NTS, NTL>,                                \
+    k_strided_copy<P, W, U, true, 1, NTS, NTL>}},
 
 // [variant][aligned][dir]
 static const KernelFn kKernels[][2][2] = {TEMPI_VARIANTS(TEMPI_KROW)};
@@ -157,28 +157,34 @@ int heuristic_variant(const Geom &g, const char *strided, const char *packed,
   (void)dir;
   const int64_t L = g.L;
   const int64_t bytes = L * g.rows;
+  // Copies this large are read from HBM: nontemporal loads let their line
+  // fills go first from the caches, ahead of the lines being written.
+  const bool big = bytes >= (8ll << 20);
   if (L >= 1024) {
-    // large copies stream past the caches: nontemporal stores
-    return bytes >= (8ll << 20) ? 2 : 0;
+    // one long run: nontemporal stores
+    if (g.rows == 1) {
+      return big ? 2 : 0;
+    }
+    return big ? 13 : 0;
   }
   if (L >= 512) {
-    return 3;
+    return big ? 14 : 3;
   }
   if (L >= 256) {
-    return 4;
+    return big ? 15 : 4;
   }
   if (L >= 128) {
-    return 5;
+    return big ? 16 : 5;
   }
   if (L >= 64) {
-    return 6;
+    return big ? 17 : 6;
   }
   const int Wa = max_aligned_word(g, strided, packed);
   if (Wa >= 16) {
     return 7;
   }
   if (Wa == 8) {
-    return 8;
+    return big ? 18 : 8;
   }
   if (Wa == 4) {
     // rows of 3 or 6 dwords move as 12-B words

@@ -33,7 +33,7 @@ namespace tempi {
 
 namespace {
 
-#define TEMPI_COUNT_V(I, P, W, U, NT) +1
+#define TEMPI_COUNT_V(I, P, W, U, NTS, NTL) +1
 constexpr int kOut = 0 TEMPI_VARIANTS(TEMPI_COUNT_V);
 #undef TEMPI_COUNT_V
 constexpr int kIn = 16, kH1 = 64, kH2 = 64;
@@ -52,11 +52,15 @@ struct Acts {
   float x[kB][kIn];
   float h1[kB][kH1];
   float h2[kB][kH2];
-  float y[kB][kOut];
-  float dy[kB][kOut];
+  float y[kB][kOut]; // outputs; replaced in place by dloss/dy in training
   float d2[kB][kH2];
   float d1[kB][kH1];
 };
+
+// parameters and activations share one workgroup's LDS; keep them within the
+// 64 KiB every CDNA part grants a workgroup
+static_assert(sizeof(Acts) + sizeof(float) * (kNParams + 3) <= 65536,
+              "MLP kernels exceed 64 KiB of LDS");
 
 // y = act(b + a W) for a[nb][K], W[K][N] row-major (row = input)
 template <int K, int N, bool RELU>
@@ -82,7 +86,8 @@ __device__ void forward(const float *__restrict__ P, Acts &a, int nb) {
   __syncthreads();
 }
 
-// dy = 2 m (y - t) / sum(m); returns (in *loss, thread 0) sum m (y-t)^2/sum m
+// dy = 2 m (y - t) / sum(m), written over y; returns (in *loss, thread 0)
+// sum m (y-t)^2 / sum m
 __device__ void loss_grad(Acts &a, const float *__restrict__ T,
                           const float *__restrict__ M,
                           const uint32_t *__restrict__ rows, int nb,
@@ -107,7 +112,7 @@ __device__ void loss_grad(Acts &a, const float *__restrict__ T,
     const uint32_t r = rows[b];
     const float m = M ? M[(size_t)r * kOut + k] : 1.f;
     const float e = a.y[b][k] - T[(size_t)r * kOut + k];
-    a.dy[b][k] = 2.f * m * e * inv;
+    a.y[b][k] = 2.f * m * e * inv; // dy, in place: this thread alone reads y[b][k]
     l += m * e * e * inv;
   }
   atomicAdd(&s_sum[1], l);
@@ -124,7 +129,7 @@ __device__ void backward_hidden(const float *__restrict__ P, Acts &a, int nb) {
     if (a.h2[b][j] > 0.f) {
 #pragma unroll
       for (int k = 0; k < kOut; ++k) {
-        acc = fmaf(a.dy[b][k], P[kOffW3 + j * kOut + k], acc);
+        acc = fmaf(a.y[b][k], P[kOffW3 + j * kOut + k], acc);
       }
     }
     a.d2[b][j] = acc;
@@ -161,10 +166,10 @@ __device__ float param_grad(const Acts &a, int p, int nb) {
     for (int b = 0; b < nb; ++b) g += a.d2[b][j];
   } else if (p < kOffB3) {
     const int j = (p - kOffW3) / kOut, k = (p - kOffW3) % kOut;
-    for (int b = 0; b < nb; ++b) g = fmaf(a.h2[b][j], a.dy[b][k], g);
+    for (int b = 0; b < nb; ++b) g = fmaf(a.h2[b][j], a.y[b][k], g);
   } else {
     const int k = p - kOffB3;
-    for (int b = 0; b < nb; ++b) g += a.dy[b][k];
+    for (int b = 0; b < nb; ++b) g += a.y[b][k];
   }
   return g;
 }

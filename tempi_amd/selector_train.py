@@ -6,9 +6,11 @@
 1. Generate seeded strided copies of the kinds the path sees: 2D/3D byte
    vectors with block lengths 1 B - 1 KiB and strides up to 4 KiB past the
    block at any base offset, halo faces/edges/corners of 3D subdomains, faces
-   of float grids, and random 1D/2D/3D blocks; packs and unpacks.
+   of float grids, and random 1D/2D/3D blocks; packs and unpacks; a fifth
+   with several objects per call, and one in twenty of 64-256 MiB.
 2. Time every kernel variant on each with the autotuner's timer
-   (tempi_time_variant: HIP events around back-to-back launches).
+   (tempi_time_variant: HIP events around back-to-back launches; copies of
+   64 MiB and more one launch at a time after a cache sweep).
 3. Train the MLP (tempi_mlp_train: the library's HIP forward / backward /
    Adam kernels) to predict each variant's log2 time relative to the best
    variant, capped at 2x; features are normalized by the training set's
@@ -36,11 +38,15 @@ import numpy as np
 import tempi_amd as ta
 
 SRC_BYTES = 3 << 30
-DST_BYTES = 64 << 20
+DST_BYTES = 256 << 20
+BIG_MIN = 64 << 20  # copies timed cold (one launch after a cache sweep)
+BIG_FRACTION = 0.05
 
 
-def _vector_case(rng, max_packed):
+def _vector_case(rng, max_packed, min_packed=1024):
     bls = [1, 3, 4, 8, 12, 16, 24, 32, 64, 128, 256, 512, 1024]
+    if min_packed > (1 << 20):
+        bls = [b for b in bls if b >= 16]
     while True:
         bl = rng.choice(bls)
         rows = 1 << rng.randint(4, 20)
@@ -55,7 +61,7 @@ def _vector_case(rng, max_packed):
             counts, strides = [bl, rows, planes], [1, stride, s2]
             span = (planes - 1) * s2 + (rows - 1) * stride + bl
         packed = int(np.prod(counts))
-        if 1024 <= packed <= max_packed and span < SRC_BYTES - 64:
+        if min_packed <= packed <= max_packed and span < SRC_BYTES - 64:
             return ta.StridedBlock.make(0, span, counts, strides), rng.randint(0, 15), span
 
 
@@ -121,7 +127,10 @@ def generate(n, seed, max_packed):
     cases = []
     for _ in range(n):
         u = rng.random()
-        if u < 0.4:
+        if rng.random() < BIG_FRACTION:
+            # HBM-sized copies, timed cold
+            sb, base, span = _vector_case(rng, DST_BYTES, BIG_MIN)
+        elif u < 0.4:
             sb, base, span = _vector_case(rng, max_packed)
         elif u < 0.6:
             sb, base, span = _halo_case(rng, max_packed)
@@ -135,7 +144,7 @@ def generate(n, seed, max_packed):
         count = 1
         if rng.random() < 0.2:
             c = rng.choice([2, 3, 4, 8])
-            if sb.extent * c + base < SRC_BYTES and sb.size() * c <= DST_BYTES:
+            if sb.extent * c + base < SRC_BYTES and sb.size() * c <= DST_BYTES - 64:
                 count = c
         cases.append((sb, base, direction, ppos, count))
     return cases
@@ -300,6 +309,16 @@ def main(argv=None):
                 "gbs_heuristic": float(B.sum() / (T2[idx, H].sum() * 1e-6) / 1e9),
                 "gbs_autotuner": float(B.sum() / (T2[idx, best].sum() * 1e-6) / 1e9),
             }
+            big = B >= BIG_MIN
+            if big.any():
+                ib = idx[big]
+                report["heldout_retimed_cold_big"] = {
+                    "copies": int(big.sum()),
+                    "gbs_model": float(B[big].sum() / (T2[ib, pm[big]].sum() * 1e-6) / 1e9),
+                    "gbs_heuristic": float(B[big].sum() / (T2[ib, H[big]].sum() * 1e-6) / 1e9),
+                    "gbs_autotuner": float(B[big].sum() / (T2[ib, best[big]].sum() * 1e-6) / 1e9),
+                    "gbs_best": float(B[big].sum() / (T2[ib].min(1).sum() * 1e-6) / 1e9),
+                }
     with open(args.report, "w") as fh:
         json.dump(report, fh, indent=1)
     print(json.dumps(report.get("heldout_retimed", report["heldout"])), flush=True)

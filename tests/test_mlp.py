@@ -11,7 +11,7 @@ import pytest
 import tempi_amd as ta
 from oracle import mlp_ref
 
-OUT = 13
+OUT = ta.lib().tempi_mlp_num_outputs()
 
 
 def fd_grad(params, x, t, m, idx, h=1e-6):
@@ -108,7 +108,7 @@ def test_model_file_round_trip_and_rejects(tmp_path):
     assert vals.size == n and (vals == params).all()
     assert L.tempi_mlp_load(path.encode()) == 0 and L.tempi_mlp_loaded() == 1
     bad = tmp_path / "bad.txt"
-    bad.write_text(open(path).read().replace("out 13", "out 12"))
+    bad.write_text(open(path).read().replace(f"out {OUT}", f"out {OUT - 1}"))
     assert L.tempi_mlp_load(str(bad).encode()) == ta.TEMPI_ERR_IO
     assert L.tempi_mlp_loaded() == 0
     trunc = tmp_path / "trunc.txt"

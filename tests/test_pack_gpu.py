@@ -127,12 +127,12 @@ SHAPES = [
 ]
 
 
-@pytest.mark.parametrize("variant", range(13))
+@pytest.mark.parametrize("variant", range(19))
 def test_every_variant_every_alignment(variant):
     """Each kernel variant, forced, on rows of 1 B - 68 KiB, with the strided
     buffer at every byte offset 0..15 class and the packed buffer at 0 and
     odd positions."""
-    assert ta.num_variants() == 13
+    assert ta.num_variants() == 19
     rng = random.Random(variant)
     for bl, counts, strides in SHAPES:
         for base in (0, 1, 4, 8, 13):
