@@ -73,6 +73,10 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="with --profile-only: also run one plain 256 MiB contiguous copy "
                          "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--profile-large-blocks", action="store_true",
+                    help="only the 256 MiB 2D packs/unpacks (64 B - 4 KiB blocks), one "
+                         "launch each after a cache sweep, then the 256 MiB contiguous "
+                         "calibration copy (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share of the pack path; --gpus {args.gpus} is not "
@@ -89,6 +93,9 @@ def main():
     assert ta.device_count() > 0, "no GPU"
     m = tmpi.get()
     stream = ta.stream()
+    if args.profile_large_blocks:
+        profile_large_blocks(ta, stream)
+        return
     n, g = 512, 3
     full = n + 2 * g
     log("generating the 518^3 float32 grid (normal(0,1), seed 2)")
@@ -285,6 +292,42 @@ def main():
     pack_batch.destroy()
     unpack_batch.destroy()
     m.finalize()
+
+
+def profile_large_blocks(ta, stream):
+    """One cold launch of each large-block copy with the library's default
+    choice, in a fixed order, then one 256 MiB contiguous copy (variant 1):
+    the kernels a counter run attributes traffic to, dispatch by dispatch."""
+    n = 256 << 20
+    src = ta.DeviceBuffer(3 << 30)
+    src.fill(3)
+    dst = ta.DeviceBuffer(n)
+    dst.fill(4)
+    for bl in (64, 256, 1024, 4096):
+        rows = n // bl
+        stride = 2 * bl + 64
+        sb = ta.StridedBlock.make(0, rows * stride, [bl, rows], [1, stride])
+        for d, name in ((ta.DIR_PACK, "pack"), (ta.DIR_UNPACK, "unpack")):
+            v = ta.select_variant(sb, 1, src.addr(), dst.addr(), d, ta.SELECT_DEFAULT)
+            ta.flush_caches()
+            ev = [ta.Event(), ta.Event()]
+            ev[0].record(stream)
+            if d == ta.DIR_PACK:
+                ta.pack_sb(sb, src.addr(), 1, dst.addr(), n, 0, v, stream)
+            else:
+                ta.unpack_sb(sb, dst.addr(), n, src.addr(), 1, 0, v, stream)
+            ev[1].record(stream)
+            ta.synchronize(stream)
+            print(json.dumps({"copy": f"{name}_bl{bl}", "variant": v,
+                              "us": round(ev[0].elapsed_ms(ev[1]) * 1e3, 2)}), flush=True)
+    src.free()
+    cal = ta.DeviceBuffer(n)
+    cal.fill(1)
+    sb1 = ta.StridedBlock.make(0, n, [n], [1])
+    ta.flush_caches()
+    ta.pack_sb(sb1, cal.addr(), 1, dst.addr(), n, 0, 1, stream)
+    ta.synchronize(stream)
+    print(json.dumps({"copy": "contiguous_256MiB", "variant": 1}), flush=True)
 
 
 def bench_large_blocks(ta, stream):

@@ -129,8 +129,10 @@ def test_commit_time_inference_drives_mpi_pack(mpi, tmp_path):
 def test_installed_model_beats_the_closed_form_rule():
     """The shipped selector, on copies it was not trained on (a seed outside
     the training and held-out ranges), moves more packed bytes per second
-    than the closed-form rule and stays close to the exhaustive autotuner's
-    best, all judged on the same per-variant timings."""
+    than the closed-form rule and about as many as the exhaustive autotuner.
+    Every variant is timed twice: the autotuner picks from the first timing,
+    and all three choices are judged on the second (the best of one noisy
+    timing, judged on that same timing, would be an unreachable bar)."""
     from tempi_amd import selector_train as st
 
     L = ta.lib()
@@ -143,17 +145,18 @@ def test_installed_model_beats_the_closed_form_rule():
     src.fill(1)
     dst = ta.DeviceBuffer(st.DST_BYTES + 4096)
     dst.fill(2)
-    print(f"timing {len(cases)} copies x {ta.num_variants()} variants", flush=True)
-    X, T, heur, nbytes = st.time_cases(cases, src, dst, log_every=0)
+    print(f"timing {len(cases)} copies x {ta.num_variants()} variants, twice", flush=True)
+    _, T1, heur, nbytes = st.time_cases(cases, src, dst, log_every=0)
+    _, T2, _, _ = st.time_cases(cases, src, dst, log_every=0)
     model = np.array([ta.select_variant(sb, count, src.addr(base), dst.addr(ppos), d,
                                         ta.SELECT_MODEL)
                       for sb, base, d, ppos, count in cases])
     rows = np.arange(len(cases))
     gbs = {
-        "model": nbytes.sum() / T[rows, model].sum() / 1e3,
-        "rule": nbytes.sum() / T[rows, heur].sum() / 1e3,
-        "best": nbytes.sum() / T.min(axis=1).sum() / 1e3,
+        "model": nbytes.sum() / T2[rows, model].sum() / 1e3,
+        "rule": nbytes.sum() / T2[rows, heur].sum() / 1e3,
+        "autotuner": nbytes.sum() / T2[rows, T1.argmin(axis=1)].sum() / 1e3,
     }
     print(gbs, flush=True)
     assert gbs["model"] >= 1.1 * gbs["rule"], gbs
-    assert gbs["model"] >= 0.85 * gbs["best"], gbs
+    assert gbs["model"] >= 0.9 * gbs["autotuner"], gbs
