@@ -7,7 +7,7 @@
    vectors with block lengths 1 B - 1 KiB and strides up to 4 KiB past the
    block at any base offset, halo faces/edges/corners of 3D subdomains, faces
    of float grids, and random 1D/2D/3D blocks; packs and unpacks; a fifth
-   with several objects per call, and one in twenty of 64-256 MiB.
+   with several objects per call, and one in ten of 64-256 MiB.
 2. Time every kernel variant on each with the autotuner's timer
    (tempi_time_variant: HIP events around back-to-back launches; copies of
    64 MiB and more one launch at a time after a cache sweep).
@@ -40,7 +40,7 @@ import tempi_amd as ta
 SRC_BYTES = 3 << 30
 DST_BYTES = 256 << 20
 BIG_MIN = 64 << 20  # copies timed cold (one launch after a cache sweep)
-BIG_FRACTION = 0.05
+BIG_FRACTION = 0.1
 
 
 def _vector_case(rng, max_packed, min_packed=1024):
@@ -50,7 +50,11 @@ def _vector_case(rng, max_packed, min_packed=1024):
     while True:
         bl = rng.choice(bls)
         rows = 1 << rng.randint(4, 20)
-        stride = bl + rng.randint(0, 4096)
+        if rng.random() < 0.6:
+            stride = bl + rng.randint(0, 4096)
+        else:
+            # dense layouts: gaps no longer than the block (or a line)
+            stride = bl + rng.choice([0, 4, 16, 64, 128, bl // 2, bl, bl + 64])
         if rng.random() < 0.5:
             counts, strides = [bl, rows], [1, stride]
             span = (rows - 1) * stride + bl
