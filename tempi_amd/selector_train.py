@@ -4,8 +4,8 @@
         --report report.json [--train 50000] [--heldout 10000]
 
 1. Generate seeded strided copies of the kinds the path sees: 2D/3D byte
-   vectors with block lengths 1 B - 1 KiB and strides up to 4 KiB past the
-   block at any base offset, halo faces/edges/corners of 3D subdomains, faces
+   vectors with block lengths 1 B - 4 KiB and strides up to 4 KiB past the
+   block (or dense: gaps of 0 - 128 B or about a block) at any base offset, halo faces/edges/corners of 3D subdomains, faces
    of float grids, and random 1D/2D/3D blocks; packs and unpacks; a fifth
    with several objects per call, and one in ten of 64-256 MiB.
 2. Time every kernel variant on each with the autotuner's timer
@@ -44,12 +44,14 @@ BIG_FRACTION = 0.1
 
 
 def _vector_case(rng, max_packed, min_packed=1024):
-    bls = [1, 3, 4, 8, 12, 16, 24, 32, 64, 128, 256, 512, 1024]
+    bls = [1, 3, 4, 8, 12, 16, 24, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+    max_log_rows = 20
     if min_packed > (1 << 20):
         bls = [b for b in bls if b >= 16]
+        max_log_rows = 22
     while True:
         bl = rng.choice(bls)
-        rows = 1 << rng.randint(4, 20)
+        rows = 1 << rng.randint(4, max_log_rows)
         if rng.random() < 0.6:
             stride = bl + rng.randint(0, 4096)
         else:
