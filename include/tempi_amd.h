@@ -129,7 +129,11 @@ int tempi_variant_info_get(int variant, tempi_variant_info *info);
 #define TEMPI_SELECT_AUTOTUNE 2  /* exhaustive timing, cached           */
 #define TEMPI_SELECT_DEFAULT 3   /* library policy (model if loaded)    */
 
-/* Variant the library would use for this copy. */
+/* Variant the library would use for this copy.  Under the default policy a
+   copy of 64 MiB or more is settled on its first run: the model's three
+   best variants and the closed-form rule's pick are each timed once, cold,
+   and the fastest is kept for the shape; before that run this returns the
+   model's pick. */
 int tempi_select_variant(const tempi_strided_block *sb, int64_t incount,
                          const void *strided_buf, const void *packed_buf,
                          int direction, int method, int *variant);
@@ -282,6 +286,7 @@ typedef struct tempi_counters {
   int64_t commits_unsupported;
   int64_t model_inferences;     /* MLP inference kernel launches        */
   int64_t autotune_runs;
+  int64_t verified_choices;     /* large copies whose model choice was timed */
 } tempi_counters;
 int tempi_counters_get(tempi_counters *out);
 int tempi_counters_reset(void);

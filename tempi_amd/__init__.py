@@ -106,6 +106,7 @@ class Counters(ctypes.Structure):
             "commits_unsupported",
             "model_inferences",
             "autotune_runs",
+            "verified_choices",
         )
     ]
 

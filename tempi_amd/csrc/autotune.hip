@@ -94,6 +94,7 @@ int flush_caches_on(int64_t bytes, hipStream_t s) {
 namespace {
 std::mutex g_tune_mu;
 std::map<std::string, Choice> g_tune_cache;
+std::map<std::string, Choice> g_verify_cache; // default policy, large copies
 } // namespace
 
 std::string shape_key_of(const Geom &g, const char *strided,
@@ -186,14 +187,64 @@ int autotune(const Geom &g, const char *strided, char *packed, int dir,
   return TEMPI_SUCCESS;
 }
 
+// The default policy for copies timed cold (see kColdBytes): the model's
+// three best variants and the closed-form rule's pick, each timed once on
+// the caller's buffers (a pack or unpack rewrites the same bytes, so the
+// trial copies change nothing), the fastest kept for the shape.  About a
+// millisecond once per shape, against 10-20 % that a wrong pick costs each
+// call of a copy this size.
+static int verify_candidates(const Geom &g, const char *strided, char *packed,
+                             int dir, int rule, hipStream_t stream,
+                             Choice *out) {
+  int cand[4];
+  int n = model_ranked(g, strided, packed, dir, 3, cand);
+  if (n == 0) {
+    return TEMPI_ERR_UNSUPPORTED;
+  }
+  if (std::find(cand, cand + n, rule) == cand + n) {
+    cand[n++] = rule;
+  }
+  float best_us = 1e30f;
+  for (int i = 0; i < n; ++i) {
+    float us = 0;
+    int err = time_config(g, strided, packed, dir, cand[i], 0, 1, stream, &us);
+    if (err) {
+      return err;
+    }
+    if (us < best_us) {
+      best_us = us;
+      out->variant = cand[i];
+      out->max_blocks = 0;
+    }
+  }
+  ++counters().verified_choices;
+  return TEMPI_SUCCESS;
+}
+
 Choice choose(const Geom &g, const char *strided, char *packed, int dir,
               int method, hipStream_t stream) {
   Choice c;
   c.variant = heuristic_variant(g, strided, packed, dir);
   if (method == TEMPI_SELECT_MODEL ||
       (method == TEMPI_SELECT_DEFAULT && model_available())) {
-    const int v =
-        model_variant(g, strided, packed, dir, shape_key_of(g, strided, packed, dir));
+    const std::string key = shape_key_of(g, strided, packed, dir);
+    if (method == TEMPI_SELECT_DEFAULT && times_cold(g)) {
+      {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_verify_cache.find(key);
+        if (it != g_verify_cache.end()) {
+          return it->second;
+        }
+      }
+      Choice v;
+      if (stream &&
+          !verify_candidates(g, strided, packed, dir, c.variant, stream, &v)) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_verify_cache[key] = v;
+        return v;
+      }
+    }
+    const int v = model_variant(g, strided, packed, dir, key);
     if (v >= 0) {
       c.variant = v;
     }

@@ -169,6 +169,7 @@ TEMPI_EXPORT int tempi_counters_get(tempi_counters *out) {
   out->commits_unsupported = c.commits_unsupported;
   out->model_inferences = c.model_inferences;
   out->autotune_runs = c.autotune_runs;
+  out->verified_choices = c.verified_choices;
   return TEMPI_SUCCESS;
 }
 
@@ -184,6 +185,7 @@ TEMPI_EXPORT int tempi_counters_reset(void) {
   c.commits_unsupported = 0;
   c.model_inferences = 0;
   c.autotune_runs = 0;
+  c.verified_choices = 0;
   return TEMPI_SUCCESS;
 }
 

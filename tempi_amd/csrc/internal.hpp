@@ -71,6 +71,7 @@ struct Counters {
   std::atomic<int64_t> commits_unsupported{0};
   std::atomic<int64_t> model_inferences{0};
   std::atomic<int64_t> autotune_runs{0};
+  std::atomic<int64_t> verified_choices{0};
 };
 Counters &counters();
 
@@ -198,6 +199,12 @@ bool model_available();
 int model_variant(const Geom &g, const char *strided, const char *packed,
                   int dir, const std::string &key); // -1: no model
 void model_precommit(const tempi_strided_block &sb);
+// the model's k best variants for this copy, best first (not cached);
+// returns how many were written (0 without a model)
+int model_ranked(const Geom &g, const char *strided, const char *packed,
+                 int dir, int k, int *out);
+// copies of at least this many packed bytes are timed cold (autotune.hip)
+bool times_cold(const Geom &g);
 
 // system performance tables (perf.hip): load <cache dir>/perf.json if present
 void perf_init();

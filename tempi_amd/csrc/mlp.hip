@@ -485,6 +485,35 @@ int model_variant(const Geom &g, const char *strided, const char *packed,
   return best;
 }
 
+int model_ranked(const Geom &g, const char *strided, const char *packed,
+                 int dir, int k, int *out) {
+  Model *m = loaded_model();
+  if (!m || k <= 0 || ensure_runtime()) {
+    return 0;
+  }
+  float f[kIn];
+  float pred[kOut];
+  copy_features(g, strided, packed, dir, f);
+  int best = -1;
+  {
+    std::lock_guard<std::mutex> lk(m->mu);
+    if (model_infer(*m, f, 1, &best, pred)) {
+      return 0;
+    }
+  }
+  int idx[kOut];
+  for (int v = 0; v < kOut; ++v) {
+    idx[v] = v;
+  }
+  std::stable_sort(idx, idx + kOut,
+                   [&](int a, int b) { return pred[a] < pred[b]; });
+  const int n = std::min(k, kOut);
+  for (int i = 0; i < n; ++i) {
+    out[i] = idx[i];
+  }
+  return n;
+}
+
 // commit-time evaluation: the type at incount 1 with 256-B aligned buffers,
 // both directions, in one inference launch; results seed the shape cache
 void model_precommit(const tempi_strided_block &sb) {

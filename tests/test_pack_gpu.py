@@ -242,6 +242,40 @@ def test_alignment_shape_sweep_sample():
         done += 1
 
 
+def test_default_policy_settles_large_copies():
+    """A copy of 64 MiB or more under the default policy: the first call
+    times a few candidates on the caller's buffers and keeps the fastest for
+    the shape, later calls reuse it; every call's bytes stay exact (packs
+    against the oracle, unpacks keep 0xAB outside the type map)."""
+    assert ta.lib().tempi_mlp_load(None) == 0
+    bl, stride, rows = 256, 576, (64 << 20) // 256
+    sb = ta.StridedBlock.make(0, (rows - 1) * stride + bl, [bl, rows], [1, stride])
+    desc = orc.sb_desc(*sb.as_tuple())
+    host = orc.splitmix_bytes(sb.extent + 8, 77)
+    expected = orc.sb_pack(desc, host, 8, 1)
+    src = ta.DeviceBuffer(host.size)
+    src.upload(host)
+    out = ta.DeviceBuffer(expected.size)
+    before = ta.counters()["verified_choices"]
+    for k in range(2):
+        out.fill(0)
+        ta.pack_sb(sb, src.addr(8), 1, out.addr(), expected.size, 0, -1)
+        ta.synchronize()
+        assert (out.download() == expected).all(), k
+        assert ta.counters()["verified_choices"] == before + 1, k
+    v = ta.select_variant(sb, 1, src.addr(8), out.addr(), ta.DIR_PACK, ta.SELECT_DEFAULT)
+    assert 0 <= v < ta.num_variants()
+    # unpack of the same shape into a 0xAB image
+    dst = ta.DeviceBuffer(host.size)
+    dst.fill(0xAB)
+    ta.unpack_sb(sb, out.addr(), expected.size, dst.addr(8), 1, 0, -1)
+    ta.synchronize()
+    ref = np.full(host.size, 0xAB, np.uint8)
+    orc.sb_unpack(desc, expected, ref, 8, 1)
+    assert (dst.download() == ref).all()
+    assert ta.counters()["verified_choices"] == before + 2
+
+
 def test_one_shot_into_mapped_host_memory(mpi):
     """MPI_Pack from device memory straight into pinned mapped host memory
     (the ONE_SHOT method) and MPI_Unpack from it."""
