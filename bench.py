@@ -235,6 +235,7 @@ def main():
         extras["vector_2d"] = bench_vector(ta, m, tc, stream)
         extras["system_tables"] = bench_system_tables(ta)
         extras["vector_sweep"] = bench_sweep(ta, m, tc, stream)
+        extras["isend_irecv_faces"] = bench_p2p(ta, m, faces, grid, grid2, host)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_grid_faces_518.json")
@@ -588,6 +589,53 @@ def bench_sweep(ta, m, tc, stream, ncases=48, max_span=8 << 30):
                 "min": round(float(np.min(fr)), 4)}
     res["per_case"] = cases
     log(f"vector sweep: { {k: v for k, v in res.items() if k != 'per_case'} }")
+    return res
+
+
+def bench_p2p(ta, m, faces, grid, grid2, host, reps=10):
+    """The three faces sent by this process to itself with MPI_Isend /
+    MPI_Irecv (all six requests outstanding, then MPI_Waitall), per staging
+    method; beside it the same exchange of host buffers through the MPI
+    library alone (its own CPU pack and unpack)."""
+    import numpy as np
+
+    res = {}
+    names = {2: "oneshot", 3: "staged"}
+    for method, name in names.items():
+        ta.lib().tempi_set_datatype_method(method)
+        ts = []
+        for r in range(reps + 2):
+            t0 = time.perf_counter()
+            reqs = []
+            for i, f in enumerate(faces):
+                reqs.append(m.irecv(grid2.addr(), 1, f["handle"], 0, 200 + i)[1])
+            for i, f in enumerate(faces):
+                reqs.append(m.isend(grid.addr(), 1, f["handle"], 0, 200 + i)[1])
+            err, _, _ = m.waitall(reqs)
+            assert err == 0
+            if r >= 2:
+                ts.append(time.perf_counter() - t0)
+        us = float(np.median(ts)) * 1e6
+        res[name] = {"us_per_exchange": round(us, 1),
+                     "packed_gbs": round(sum(f["size"] for f in faces) / us / 1e3, 2)}
+    ta.lib().tempi_set_datatype_method(-1)
+    dst = np.zeros_like(host)
+    ts = []
+    for r in range(4):
+        t0 = time.perf_counter()
+        reqs = []
+        for i, f in enumerate(faces):
+            reqs.append(m.irecv(dst.ctypes.data, 1, f["handle"], 0, 300 + i)[1])
+        for i, f in enumerate(faces):
+            reqs.append(m.isend(host.ctypes.data, 1, f["handle"], 0, 300 + i)[1])
+        err, _, _ = m.waitall(reqs)
+        assert err == 0
+        if r >= 1:
+            ts.append(time.perf_counter() - t0)
+    us = float(np.median(ts)) * 1e6
+    res["mpi_library_host_buffers"] = {"us_per_exchange": round(us, 1),
+                                       "packed_gbs": round(sum(f["size"] for f in faces) / us / 1e3, 2)}
+    log(f"isend/irecv of the faces: {res}")
     return res
 
 

@@ -53,6 +53,36 @@ int MPI_Pack(const void *inbuf, int incount, MPI_Datatype datatype,
    written */
 int MPI_Unpack(const void *inbuf, int insize, int *position, void *outbuf,
                int outcount, MPI_Datatype datatype, MPI_Comm comm);
+/* Point-to-point transfers of device-accessible buffers whose type has a
+   non-contiguous strided form travel as packed bytes (MPI_PACKED), staged
+   in device memory (DEVICE, needs a GPU-aware MPI library), pinned host
+   memory the kernel writes directly (ONESHOT) or device memory copied to
+   pinned host memory (STAGED); anything else goes to the library as it is.
+   reference: src/send.cpp:10, src/internal/send.cpp:20-50,
+   src/internal/sender.cpp:88-328 */
+int MPI_Send(const void *buf, int count, MPI_Datatype datatype, int dest,
+             int tag, MPI_Comm comm);
+/* reference: src/recv.cpp:19-52 */
+int MPI_Recv(void *buf, int count, MPI_Datatype datatype, int source, int tag,
+             MPI_Comm comm, MPI_Status *status);
+/* reference: src/isend.cpp:11, src/internal/isend.cpp:15-45,
+   src/internal/async_operation.cpp:70-206 -- the caller receives a request
+   handle of this library's, completed by MPI_Wait / MPI_Waitall / MPI_Test */
+int MPI_Isend(const void *buf, int count, MPI_Datatype datatype, int dest,
+              int tag, MPI_Comm comm, MPI_Request *request);
+/* reference: src/irecv.cpp:11, src/internal/irecv.cpp:15-45,
+   src/internal/async_operation.cpp:209-331 */
+int MPI_Irecv(void *buf, int count, MPI_Datatype datatype, int source, int tag,
+              MPI_Comm comm, MPI_Request *request);
+/* reference: src/wait.cpp:11, src/internal/async_operation.cpp:460-477 */
+int MPI_Wait(MPI_Request *request, MPI_Status *status);
+/* no active counterpart in the reference (its version is compiled out,
+   async_operation.cpp:479-511): completes this library's requests together
+   with the MPI library's own */
+int MPI_Waitall(int count, MPI_Request array_of_requests[],
+                MPI_Status array_of_statuses[]);
+/* no counterpart in the reference: tests this library's requests */
+int MPI_Test(MPI_Request *request, int *flag, MPI_Status *status);
 
 /* ------------------------------------------------------------------------ */
 /* 2. tempi_* entry points                                                  */
@@ -287,6 +317,10 @@ typedef struct tempi_counters {
   int64_t model_inferences;     /* MLP inference kernel launches        */
   int64_t autotune_runs;
   int64_t verified_choices;     /* large copies whose model choice was timed */
+  int64_t p2p_device;           /* sends / receives of device data by method */
+  int64_t p2p_oneshot;
+  int64_t p2p_staged;
+  int64_t p2p_library;          /* point-to-point calls forwarded as they are */
 } tempi_counters;
 int tempi_counters_get(tempi_counters *out);
 int tempi_counters_reset(void);
@@ -363,6 +397,13 @@ int tempi_perf_model(int method, int colocated, int64_t bytes,
    cached per (colocated, bytes, block_length).  reference:
    SendRecvND::send, src/internal/sender.cpp:251-273. */
 int tempi_choose_method(int colocated, int64_t bytes, int64_t block_length);
+/* Force the method of later sends and receives of non-contiguous device data
+   (TEMPI_METHOD_DEVICE / ONESHOT / STAGED; TEMPI_METHOD_UNKNOWN: choose by
+   the model; -1: back to the TEMPI_DATATYPE_* environment).  Returns the
+   previous setting.  The runtime form of the reference's
+   TEMPI_DATATYPE_{ONESHOT,DEVICE,AUTO} variables (src/internal/env.cpp:
+   73-81), for tools and tests. */
+int tempi_set_datatype_method(int method);
 
 #ifdef __cplusplus
 }

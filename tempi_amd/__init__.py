@@ -107,6 +107,10 @@ class Counters(ctypes.Structure):
             "model_inferences",
             "autotune_runs",
             "verified_choices",
+            "p2p_device",
+            "p2p_oneshot",
+            "p2p_staged",
+            "p2p_library",
         )
     ]
 

@@ -30,6 +30,8 @@ SOURCES = [
     "batch.hip",
     "mlp.hip",
     "perf.hip",
+    "pool.cpp",
+    "p2p.cpp",
 ]
 
 ARCH = os.environ.get("TEMPI_ARCH", "gfx950")

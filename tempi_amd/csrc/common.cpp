@@ -5,6 +5,7 @@
 #include <dlfcn.h>
 
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -78,6 +79,29 @@ void read_environment() {
   } else {
     e.cache_dir = "/var/tmp";
   }
+  // send methods for device data (reference: src/internal/env.cpp:73-86)
+  e.datatype_method = TEMPI_METHOD_UNKNOWN;
+  if (std::getenv("TEMPI_DATATYPE_ONESHOT")) {
+    e.datatype_method = TEMPI_METHOD_ONESHOT;
+  }
+  if (std::getenv("TEMPI_DATATYPE_DEVICE")) {
+    e.datatype_method = TEMPI_METHOD_DEVICE;
+  }
+  if (std::getenv("TEMPI_DATATYPE_STAGED")) {
+    e.datatype_method = TEMPI_METHOD_STAGED;
+  }
+  if (std::getenv("TEMPI_DATATYPE_AUTO")) {
+    e.datatype_method = TEMPI_METHOD_UNKNOWN;
+  }
+  e.contiguous_method = TEMPI_METHOD_UNKNOWN;
+  e.contiguous_auto = false;
+  if (std::getenv("TEMPI_CONTIGUOUS_STAGED")) {
+    e.contiguous_method = TEMPI_METHOD_STAGED;
+  }
+  if (std::getenv("TEMPI_CONTIGUOUS_AUTO")) {
+    e.contiguous_auto = true;
+  }
+  e.gpu_aware_mpi = nullptr != std::getenv("TEMPI_MPI_GPU_AWARE");
   if (const char *m = std::getenv("TEMPI_MODEL")) {
     e.model_path = m;
   } else {
@@ -139,6 +163,13 @@ const RealMpi &real_mpi() {
     TEMPI_RESOLVE(Type_free, MPI_Type_free);
     TEMPI_RESOLVE(Pack, MPI_Pack);
     TEMPI_RESOLVE(Unpack, MPI_Unpack);
+    TEMPI_RESOLVE(Send, MPI_Send);
+    TEMPI_RESOLVE(Recv, MPI_Recv);
+    TEMPI_RESOLVE(Isend, MPI_Isend);
+    TEMPI_RESOLVE(Irecv, MPI_Irecv);
+    TEMPI_RESOLVE(Wait, MPI_Wait);
+    TEMPI_RESOLVE(Waitall, MPI_Waitall);
+    TEMPI_RESOLVE(Test, MPI_Test);
 #undef TEMPI_RESOLVE
   });
   return r;
@@ -170,6 +201,10 @@ TEMPI_EXPORT int tempi_counters_get(tempi_counters *out) {
   out->model_inferences = c.model_inferences;
   out->autotune_runs = c.autotune_runs;
   out->verified_choices = c.verified_choices;
+  out->p2p_device = c.p2p_device;
+  out->p2p_oneshot = c.p2p_oneshot;
+  out->p2p_staged = c.p2p_staged;
+  out->p2p_library = c.p2p_library;
   return TEMPI_SUCCESS;
 }
 
@@ -186,6 +221,10 @@ TEMPI_EXPORT int tempi_counters_reset(void) {
   c.model_inferences = 0;
   c.autotune_runs = 0;
   c.verified_choices = 0;
+  c.p2p_device = 0;
+  c.p2p_oneshot = 0;
+  c.p2p_staged = 0;
+  c.p2p_library = 0;
   return TEMPI_SUCCESS;
 }
 
@@ -210,6 +249,12 @@ TEMPI_EXPORT int64_t tempi_mpi_constant(const char *name, int *ok) {
       E(MPI_DATATYPE_NULL), E(MPI_ORDER_C),       E(MPI_ORDER_FORTRAN),
       E(MPI_SUCCESS),       E(MPI_ERR_TRUNCATE),  E(MPI_ERRORS_RETURN),
       E(MPI_ERRORS_ARE_FATAL), E(MPI_THREAD_SINGLE), E(MPI_THREAD_MULTIPLE),
+      E(MPI_REQUEST_NULL),  E(MPI_ANY_SOURCE),    E(MPI_ANY_TAG),
+      E(MPI_PROC_NULL),     E(MPI_ERR_COUNT),
+      {"sizeof(MPI_Status)", (int64_t)sizeof(MPI_Status)},
+      {"offsetof(MPI_Status,MPI_SOURCE)", (int64_t)offsetof(MPI_Status, MPI_SOURCE)},
+      {"offsetof(MPI_Status,MPI_TAG)", (int64_t)offsetof(MPI_Status, MPI_TAG)},
+      {"offsetof(MPI_Status,MPI_ERROR)", (int64_t)offsetof(MPI_Status, MPI_ERROR)},
   };
 #undef E
   for (const Entry &e : table) {

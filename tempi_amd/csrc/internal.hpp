@@ -49,6 +49,13 @@ struct Env {
   bool disable = false;        // TEMPI_DISABLE: pure pass-through
   bool no_pack = false;        // TEMPI_NO_PACK: MPI_Pack/Unpack -> library
   bool no_type_commit = false; // TEMPI_NO_TYPE_COMMIT: no canonicalization
+  // TEMPI_DATATYPE_{ONESHOT,DEVICE,STAGED,AUTO}: how non-contiguous device
+  // data is sent (TEMPI_METHOD_*; UNKNOWN = choose by the system model)
+  int datatype_method = TEMPI_METHOD_UNKNOWN;
+  // TEMPI_CONTIGUOUS_{STAGED,AUTO}: contiguous device data (default: library)
+  int contiguous_method = TEMPI_METHOD_UNKNOWN;
+  bool contiguous_auto = false;
+  bool gpu_aware_mpi = false;  // TEMPI_MPI_GPU_AWARE: library takes device pointers
   int select = TEMPI_SELECT_DEFAULT; // TEMPI_SELECT=heuristic|model|autotune
   std::string cache_dir;       // TEMPI_CACHE_DIR (autotune cache, model)
   std::string model_path;      // TEMPI_MODEL (MLP weights file)
@@ -72,6 +79,10 @@ struct Counters {
   std::atomic<int64_t> model_inferences{0};
   std::atomic<int64_t> autotune_runs{0};
   std::atomic<int64_t> verified_choices{0};
+  std::atomic<int64_t> p2p_device{0};  // sends/receives by method
+  std::atomic<int64_t> p2p_oneshot{0};
+  std::atomic<int64_t> p2p_staged{0};
+  std::atomic<int64_t> p2p_library{0}; // forwarded to the MPI library
 };
 Counters &counters();
 
@@ -89,6 +100,16 @@ struct RealMpi {
               MPI_Comm) = nullptr;
   int (*Unpack)(const void *, int, int *, void *, int, MPI_Datatype,
                 MPI_Comm) = nullptr;
+  int (*Send)(const void *, int, MPI_Datatype, int, int, MPI_Comm) = nullptr;
+  int (*Recv)(void *, int, MPI_Datatype, int, int, MPI_Comm,
+              MPI_Status *) = nullptr;
+  int (*Isend)(const void *, int, MPI_Datatype, int, int, MPI_Comm,
+               MPI_Request *) = nullptr;
+  int (*Irecv)(void *, int, MPI_Datatype, int, int, MPI_Comm,
+               MPI_Request *) = nullptr;
+  int (*Wait)(MPI_Request *, MPI_Status *) = nullptr;
+  int (*Waitall)(int, MPI_Request *, MPI_Status *) = nullptr;
+  int (*Test)(MPI_Request *, int *, MPI_Status *) = nullptr;
 };
 // resolves on first call; aborts with a message if a symbol is missing
 const RealMpi &real_mpi();
@@ -208,6 +229,26 @@ bool times_cold(const Geom &g);
 
 // system performance tables (perf.hip): load <cache dir>/perf.json if present
 void perf_init();
+// modelled seconds of a method (perf.hip); +inf when a table is missing
+double perf_model_seconds(int method, bool colocated, int64_t bytes,
+                          int64_t block_length);
+
+// the per-type record cache (interpose.cpp)
+bool type_lookup(MPI_Datatype t, Canon *out);
+// pack (dir 0) or unpack `count` objects on `stream` with the library's
+// variant choice (variant < 0) or the given one; device addresses
+int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
+             char *packed, int dir, int variant, hipStream_t stream);
+
+// cached staging buffers (pool.cpp): device memory, or pinned host memory
+// mapped into the device address space
+void *pool_acquire(bool host, size_t bytes);
+void pool_release(bool host, void *p);
+void pool_trim();
+
+// point-to-point state (p2p.cpp)
+void p2p_init();     // at MPI_Init: which ranks share a node
+void p2p_finalize(); // at MPI_Finalize: completes or reports pending work
 
 // exhaustive search; fills best and, when given, the best time per variant
 int autotune(const Geom &g, const char *strided, char *packed, int dir,

@@ -30,7 +30,9 @@ TypeCache &type_cache() {
 
 int64_t key_of(MPI_Datatype t) { return (int64_t)(intptr_t)t; }
 
-bool lookup(MPI_Datatype t, Canon *out) {
+} // namespace
+
+bool type_lookup(MPI_Datatype t, Canon *out) {
   TypeCache &c = type_cache();
   std::lock_guard<std::mutex> lk(c.mu);
   auto it = c.map.find(key_of(t));
@@ -40,6 +42,10 @@ bool lookup(MPI_Datatype t, Canon *out) {
   *out = it->second.canon;
   return true;
 }
+
+namespace {
+bool lookup(MPI_Datatype t, Canon *out) { return type_lookup(t, out); }
+} // namespace
 
 // Shared body of pack and unpack on device-addressable buffers.
 // `strided` / `packed` are device addresses of the strided buffer origin and
@@ -63,8 +69,6 @@ int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
   return launch_copy(g, strided, packed, dir, c.variant, stream, c.max_blocks);
 }
 
-} // namespace
-
 } // namespace tempi
 
 using namespace tempi;
@@ -84,6 +88,7 @@ TEMPI_EXPORT int MPI_Init(int *argc, char ***argv) {
     (void)ensure_runtime();
   }
   perf_init();
+  p2p_init();
   return err;
 }
 
@@ -98,6 +103,7 @@ TEMPI_EXPORT int MPI_Init_thread(int *argc, char ***argv, int required,
     (void)ensure_runtime();
   }
   perf_init();
+  p2p_init();
   return err;
 }
 
@@ -109,6 +115,7 @@ TEMPI_EXPORT int MPI_Finalize(void) {
       std::lock_guard<std::mutex> lk(c.mu);
       c.map.clear();
     }
+    p2p_finalize();
     finalize_runtime();
   }
   return real_mpi().Finalize();

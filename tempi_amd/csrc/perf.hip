@@ -739,6 +739,12 @@ int check_table_id(int t) {
 
 } // namespace
 
+double perf_model_seconds(int method, bool colocated, int64_t bytes,
+                          int64_t block_length) {
+  std::lock_guard<std::mutex> lk(g_perf_mu);
+  return model_locked(method, colocated, bytes, block_length);
+}
+
 // load <cache dir>/perf.json if present (reference: system::init at
 // MPI_Init, measure_system.cu:32)
 void perf_init() {

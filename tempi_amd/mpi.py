@@ -42,7 +42,9 @@ class Mpi:
     """ctypes view of the MPI library as seen from libtempi_amd.so.
 
     ``interposed`` calls (MPI_Pack, MPI_Unpack, MPI_Type_commit, MPI_Type_free,
-    MPI_Init, MPI_Finalize) resolve to libtempi_amd.so's definitions; the
+    MPI_Init, MPI_Finalize, and the point-to-point calls MPI_Send, MPI_Recv,
+    MPI_Isend, MPI_Irecv, MPI_Wait, MPI_Waitall, MPI_Test) resolve to
+    libtempi_amd.so's definitions; the
     ``library_*`` calls go straight to the underlying libmpi (the reference
     test's oracle, test/pack_unpack.cpp:64-66).
     """
@@ -103,6 +105,18 @@ class Mpi:
         t.MPI_Comm_set_errhandler.argtypes = [H, H]
         t.MPI_Initialized.argtypes = [Ip]
         t.MPI_Finalized.argtypes = [Ip]
+        # point to point (interposed) and what completes it
+        Rp = ctypes.POINTER(self.H)
+        t.MPI_Send.argtypes = [P, I, H, I, I, H]
+        t.MPI_Recv.argtypes = [P, I, H, I, I, H, P]
+        t.MPI_Isend.argtypes = [P, I, H, I, I, H, Rp]
+        t.MPI_Irecv.argtypes = [P, I, H, I, I, H, Rp]
+        t.MPI_Wait.argtypes = [Rp, P]
+        t.MPI_Waitall.argtypes = [I, Rp, P]
+        t.MPI_Test.argtypes = [Rp, Ip, P]
+        t.MPI_Get_count.argtypes = [P, H, Ip]
+        self.status_size = self.const("sizeof(MPI_Status)")
+        self.REQUEST_NULL = self.const("MPI_REQUEST_NULL")
 
     # -- lifetime ------------------------------------------------------------
     def init(self):
@@ -238,6 +252,67 @@ class Mpi:
         err = L.MPI_Unpack(inbuf, insize, ctypes.byref(pos), outbuf, outcount, h,
                            self.COMM_WORLD)
         return err, pos.value
+
+    # -- point to point -------------------------------------------------------
+    def _status_dict(self, raw, h=None):
+        def field(name):
+            off = self.const(f"offsetof(MPI_Status,{name})")
+            return int.from_bytes(bytes(raw[off:off + 4]), "little", signed=True)
+
+        out = {"source": field("MPI_SOURCE"), "tag": field("MPI_TAG"),
+               "error": field("MPI_ERROR")}
+        if h is not None:
+            n = ctypes.c_int()
+            self.tempi.MPI_Get_count(raw, h, ctypes.byref(n))
+            out["count"] = n.value
+        return out
+
+    def send(self, buf, count, h, dest, tag):
+        return self.tempi.MPI_Send(buf, count, h, dest, tag, self.COMM_WORLD)
+
+    def recv(self, buf, count, h, source, tag):
+        """MPI_Recv; returns (error code, status dict)."""
+        raw = (ctypes.c_char * self.status_size)()
+        err = self.tempi.MPI_Recv(buf, count, h, source, tag, self.COMM_WORLD, raw)
+        return err, self._status_dict(raw, h)
+
+    def isend(self, buf, count, h, dest, tag):
+        """MPI_Isend; returns (error code, request handle)."""
+        r = self.H()
+        err = self.tempi.MPI_Isend(buf, count, h, dest, tag, self.COMM_WORLD, ctypes.byref(r))
+        return err, r.value
+
+    def irecv(self, buf, count, h, source, tag):
+        r = self.H()
+        err = self.tempi.MPI_Irecv(buf, count, h, source, tag, self.COMM_WORLD, ctypes.byref(r))
+        return err, r.value
+
+    def wait(self, request, h=None):
+        """MPI_Wait; returns (error code, status dict, request after)."""
+        r = self.H(request)
+        raw = (ctypes.c_char * self.status_size)()
+        err = self.tempi.MPI_Wait(ctypes.byref(r), raw)
+        return err, self._status_dict(raw, h), r.value
+
+    def waitall(self, requests, types=None):
+        """MPI_Waitall; returns (error code, [status dict], [request after])."""
+        n = len(requests)
+        arr = (self.H * max(n, 1))(*requests)
+        raw = (ctypes.c_char * (self.status_size * max(n, 1)))()
+        err = self.tempi.MPI_Waitall(n, arr, raw)
+        sts = []
+        for i in range(n):
+            one = (ctypes.c_char * self.status_size).from_buffer(raw, i * self.status_size)
+            sts.append(self._status_dict(one, types[i] if types else None))
+        return err, sts, list(arr[:n])
+
+    def test(self, request, h=None):
+        """MPI_Test; returns (error code, flag, status dict, request after)."""
+        r = self.H(request)
+        flag = ctypes.c_int()
+        raw = (ctypes.c_char * self.status_size)()
+        err = self.tempi.MPI_Test(ctypes.byref(r), ctypes.byref(flag), raw)
+        return err, flag.value, self._status_dict(raw, h), r.value
 
     def strided_block(self, h):
         """(StridedBlock, size) of a type committed through the interposer, or

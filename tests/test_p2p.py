@@ -1,0 +1,50 @@
+"""Point-to-point calls on the CPU: host buffers go to the MPI library as
+they are (reference: src/internal/send.cpp:24-36, isend.cpp:20-26), with
+this process sending to itself; and the argument checks of the method
+override."""
+import ctypes
+
+import numpy as np
+
+import tempi_amd as ta
+import typecatalog as tc
+from oracle import oracle as orc
+
+
+def test_host_buffers_take_the_library_path(mpi):
+    spec = ("vector", 64, 3, 7, tc.BYTE)
+    h = mpi.type_from_spec(spec)
+    o = orc.OracleType(spec)
+    src = orc.splitmix_bytes(o.extent(), 11)
+    dst = np.full(o.extent(), 0xAB, np.uint8)
+    before = ta.counters()["p2p_library"]
+    err, rr = mpi.irecv(dst.ctypes.data, 1, h, 0, 5)
+    assert err == 0
+    err, sr = mpi.isend(src.ctypes.data, 1, h, 0, 5)
+    assert err == 0
+    err, sts, reqs = mpi.waitall([sr, rr], [h, h])
+    assert err == 0 and all(r == mpi.REQUEST_NULL for r in reqs)
+    assert sts[1]["count"] == 1 and sts[1]["source"] == 0 and sts[1]["tag"] == 5
+    expected = np.full(o.extent(), 0xAB, np.uint8)
+    o.unpack(o.pack(src, 0, 1), expected, 0, 1)
+    assert (dst == expected).all()
+    assert ta.counters()["p2p_library"] == before + 2
+    # blocking send into a receive posted first, then MPI_Test to completion
+    dst[:] = 0xAB
+    err, rr = mpi.irecv(dst.ctypes.data, 1, h, 0, 6)
+    assert mpi.send(src.ctypes.data, 1, h, 0, 6) == 0
+    flag = 0
+    for _ in range(10000):
+        err, flag, st, rr = mpi.test(rr, h)
+        if flag:
+            break
+    assert flag and st["count"] == 1 and rr == mpi.REQUEST_NULL
+    assert (dst == expected).all()
+    mpi.free(h)
+
+
+def test_method_override_arguments():
+    L = ta.lib()
+    assert L.tempi_set_datatype_method(99) == ta.TEMPI_ERR_ARG
+    prev = L.tempi_set_datatype_method(2)  # ONESHOT
+    assert L.tempi_set_datatype_method(prev) == 2

@@ -1,0 +1,174 @@
+"""Point-to-point transfers of non-contiguous device data on the GPU, this
+process sending to itself: every staging method, blocking and non-blocking
+calls, completion by MPI_Wait / MPI_Waitall / MPI_Test, statuses, short
+receives.  Received images are checked byte for byte against the oracle
+(the type map unpacked into a 0xAB-filled buffer)."""
+import numpy as np
+import pytest
+
+import tempi_amd as ta
+import typecatalog as tc
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+DEVICE, ONESHOT, STAGED, AUTO = 1, 2, 3, 0
+COUNTER = {ONESHOT: "p2p_oneshot", STAGED: "p2p_staged"}
+
+
+@pytest.fixture(autouse=True)
+def method_default():
+    assert ta.device_count() > 0
+    yield
+    ta.lib().tempi_set_datatype_method(-1)
+
+
+def setup(mpi, spec, count, seed):
+    h = mpi.type_from_spec(spec)
+    o = orc.OracleType(spec)
+    n = o.extent() * count
+    host = orc.splitmix_bytes(n, seed)
+    src = ta.DeviceBuffer(n)
+    src.upload(host)
+    dst = ta.DeviceBuffer(n)
+    dst.fill(0xAB)
+    expected = np.full(n, 0xAB, np.uint8)
+    o.unpack(o.pack(host, 0, count), expected, 0, count)
+    return h, src, dst, expected
+
+
+FACE = tc.grid_face_subarray(32, 3, "x")  # 12-B rows of a 38^3 float grid
+
+
+@pytest.mark.parametrize("method", [ONESHOT, STAGED, AUTO])
+def test_isend_irecv_self(mpi, method):
+    ta.lib().tempi_set_datatype_method(method)
+    h, src, dst, expected = setup(mpi, FACE, 2, 3)
+    c0 = ta.counters()
+    err, rr = mpi.irecv(dst.addr(), 2, h, 0, 11)
+    assert err == 0
+    err, sr = mpi.isend(src.addr(), 2, h, 0, 11)
+    assert err == 0
+    err, sts, reqs = mpi.waitall([rr, sr], [h, h])
+    assert err == 0 and reqs == [mpi.REQUEST_NULL] * 2
+    assert sts[0]["count"] == 2 and sts[0]["source"] == 0 and sts[0]["tag"] == 11
+    assert (dst.download() == expected).all()
+    c1 = ta.counters()
+    if method in COUNTER:
+        assert c1[COUNTER[method]] == c0[COUNTER[method]] + 2
+    assert c1["p2p_library"] == c0["p2p_library"]
+    mpi.free(h)
+
+
+@pytest.mark.parametrize("method", [ONESHOT, STAGED])
+def test_blocking_send_and_recv_self(mpi, method):
+    ta.lib().tempi_set_datatype_method(method)
+    spec = ("vector", 4096, 48, 200, tc.BYTE)
+    h, src, dst, expected = setup(mpi, spec, 1, 5)
+    # blocking send matched by a receive posted first
+    err, rr = mpi.irecv(dst.addr(), 1, h, 0, 21)
+    assert err == 0
+    assert mpi.send(src.addr(), 1, h, 0, 21) == 0
+    err, st, rr = mpi.wait(rr, h)
+    assert err == 0 and st["count"] == 1 and rr == mpi.REQUEST_NULL
+    assert (dst.download() == expected).all()
+    # blocking receive of a send issued first (still packing when it starts)
+    dst.fill(0xAB)
+    err, sr = mpi.isend(src.addr(), 1, h, 0, 22)
+    assert err == 0
+    err, st = mpi.recv(dst.addr(), 1, h, 0, 22)
+    assert err == 0 and st["count"] == 1 and st["tag"] == 22
+    err, _, sr = mpi.wait(sr)
+    assert err == 0
+    assert (dst.download() == expected).all()
+    mpi.free(h)
+
+
+def test_completion_by_test(mpi):
+    ta.lib().tempi_set_datatype_method(ONESHOT)
+    h, src, dst, expected = setup(mpi, FACE, 1, 7)
+    err, rr = mpi.irecv(dst.addr(), 1, h, 0, 31)
+    err, sr = mpi.isend(src.addr(), 1, h, 0, 31)
+    done = {}
+    for _ in range(200000):
+        for name, r in (("recv", rr), ("send", sr)):
+            if name in done:
+                continue
+            err, flag, st, after = mpi.test(r, h)
+            assert err == 0
+            if flag:
+                done[name] = st
+                assert after == mpi.REQUEST_NULL
+        if len(done) == 2:
+            break
+    assert len(done) == 2, "MPI_Test never completed"
+    assert done["recv"]["count"] == 1
+    assert (dst.download() == expected).all()
+    mpi.free(h)
+
+
+def test_short_receive_unpacks_what_arrived(mpi):
+    """A receive posted for 3 objects that gets 1: the status says 1 and only
+    the first object's type map is written."""
+    ta.lib().tempi_set_datatype_method(STAGED)
+    spec = ("vector", 100, 8, 24, tc.BYTE)
+    h = mpi.type_from_spec(spec)
+    o = orc.OracleType(spec)
+    host = orc.splitmix_bytes(o.extent() * 3, 9)
+    src = ta.DeviceBuffer(host.size)
+    src.upload(host)
+    dst = ta.DeviceBuffer(host.size)
+    dst.fill(0xAB)
+    err, rr = mpi.irecv(dst.addr(), 3, h, 0, 41)
+    err, sr = mpi.isend(src.addr(), 1, h, 0, 41)
+    err, sts, _ = mpi.waitall([sr, rr], [h, h])
+    assert err == 0 and sts[1]["count"] == 1
+    expected = np.full(host.size, 0xAB, np.uint8)
+    o.unpack(o.pack(host, 0, 1), expected, 0, 1)
+    assert (dst.download() == expected).all()
+    mpi.free(h)
+
+
+def test_device_method_without_gpu_aware_library_stages(mpi):
+    ta.lib().tempi_set_datatype_method(DEVICE)
+    h, src, dst, expected = setup(mpi, FACE, 1, 13)
+    c0 = ta.counters()["p2p_staged"]
+    err, rr = mpi.irecv(dst.addr(), 1, h, 0, 51)
+    err, sr = mpi.isend(src.addr(), 1, h, 0, 51)
+    err, _, _ = mpi.waitall([rr, sr])
+    assert err == 0
+    assert ta.counters()["p2p_staged"] == c0 + 2
+    assert (dst.download() == expected).all()
+    mpi.free(h)
+
+
+def test_grid_faces_exchange_self(mpi):
+    """The six faces of a 134^3 float64 subdomain (radius 3) sent to itself
+    with every request outstanding at once, as a halo exchange posts them."""
+    halos = [(d, tc.halo_type(3, (128, 128, 128), d, 8))
+             for d in [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]]
+    n = 134 ** 3 * 8
+    host = orc.splitmix_bytes(n, 17)
+    src = ta.DeviceBuffer(n)
+    src.upload(host)
+    dst = ta.DeviceBuffer(n)
+    dst.fill(0xAB)
+    expected = np.full(n, 0xAB, np.uint8)
+    handles, reqs = [], []
+    for i, (d, spec) in enumerate(halos):
+        h = mpi.type_from_spec(spec)
+        handles.append(h)
+        o = orc.OracleType(spec)
+        o.unpack(o.pack(host, 0, 1), expected, 0, 1)
+        err, r = mpi.irecv(dst.addr(), 1, h, 0, 100 + i)
+        assert err == 0
+        reqs.append(r)
+    for i, h in enumerate(handles):
+        err, r = mpi.isend(src.addr(), 1, h, 0, 100 + i)
+        assert err == 0
+        reqs.append(r)
+    err, sts, after = mpi.waitall(reqs)
+    assert err == 0 and after == [mpi.REQUEST_NULL] * len(reqs)
+    assert (dst.download() == expected).all()
+    for h in handles:
+        mpi.free(h)
