@@ -60,6 +60,9 @@ static void test_type(const char *name, MPI_Datatype t, int count) {
   int p = 0;
   MPI_Unpack(exp.data(), packed, &p, uexp.data(), count, t, MPI_COMM_WORLD);
   CHECK_HIP(hipMemset(ddst, 0xAB, span));
+  // the library works on its own non-blocking stream, as the reference's
+  // kernStream: the caller finishes producing its buffers first
+  CHECK_HIP(hipDeviceSynchronize());
   p = 0;
   MPI_Unpack(dpk, packed, &p, ddst, count, t, MPI_COMM_WORLD);
   CHECK_HIP(hipMemcpy(ugot.data(), ddst, span, hipMemcpyDeviceToHost));
