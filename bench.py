@@ -236,6 +236,7 @@ def main():
         extras["system_tables"] = bench_system_tables(ta)
         extras["vector_sweep"] = bench_sweep(ta, m, tc, stream)
         extras["isend_irecv_faces"] = bench_p2p(ta, m, faces, grid, grid2, host)
+        extras["neighbor_alltoallw_halo"] = bench_neighbor(ta, m, tc)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_grid_faces_518.json")
@@ -636,6 +637,58 @@ def bench_p2p(ta, m, faces, grid, grid2, host, reps=10):
     res["mpi_library_host_buffers"] = {"us_per_exchange": round(us, 1),
                                        "packed_gbs": round(sum(f["size"] for f in faces) / us / 1e3, 2)}
     log(f"isend/irecv of the faces: {res}")
+    return res
+
+
+def bench_neighbor(ta, m, tc, reps=20):
+    """A periodic halo exchange of one 128^3 float64 subdomain (radius 3)
+    whose 26 neighbours are all this process: MPI_Neighbor_alltoallw (one
+    batched pack, the library's alltoallv, one batched unpack) per staging
+    method, against the reference's scheme of one MPI_Isend / MPI_Irecv per
+    neighbour through the same library."""
+    import numpy as np
+
+    dirs = tc.halo_directions()
+    lcr = (128, 128, 128)
+    st = [m.type_from_spec(tc.halo_type(3, lcr, d, 8)) for d in dirs]
+    rt = [m.type_from_spec(tc.halo_type(3, lcr, tuple(-x for x in d), 8, exterior=True))
+          for d in dirs]
+    k = len(dirs)
+    comm = m.dist_graph([0] * k, [0] * k)
+    n = 134 ** 3 * 8
+    src, dst = ta.DeviceBuffer(n), ta.DeviceBuffer(n)
+    src.fill(1)
+    dst.fill(2)
+    packed = sum(m.strided_block(h)[1] for h in st)
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e6
+
+    def alltoallw():
+        assert m.neighbor_alltoallw(src.addr(), [1] * k, [0] * k, st, dst.addr(), [1] * k,
+                                    [0] * k, rt, comm) == 0
+
+    def isend_irecv():
+        reqs = [m.irecv(dst.addr(), 1, h, 0, 400 + i)[1] for i, h in enumerate(rt)]
+        reqs += [m.isend(src.addr(), 1, h, 0, 400 + i)[1] for i, h in enumerate(st)]
+        assert m.waitall(reqs)[0] == 0
+
+    res = {"neighbours": k, "packed_bytes": packed}
+    for method, name in ((2, "oneshot"), (3, "staged")):
+        ta.lib().tempi_set_datatype_method(method)
+        res[f"alltoallw_{name}_us"] = round(timed(alltoallw), 1)
+        res[f"isend_irecv_{name}_us"] = round(timed(isend_irecv), 1)
+    ta.lib().tempi_set_datatype_method(-1)
+    m.comm_free(comm)
+    for h in st + rt:
+        m.free(h)
+    log(f"neighbor alltoallw: {res}")
     return res
 
 

@@ -83,6 +83,18 @@ int MPI_Waitall(int count, MPI_Request array_of_requests[],
                 MPI_Status array_of_statuses[]);
 /* no counterpart in the reference: tests this library's requests */
 int MPI_Test(MPI_Request *request, int *flag, MPI_Status *status);
+/* reference: src/neighbor_alltoallw.cpp:11-18,
+   src/internal/neighbor_alltoallw.cpp:19-79 (one Isend/Irecv per
+   neighbour).  Here: one batched pack launch for all outgoing blocks, the
+   library's MPI_Neighbor_alltoallv of the packed bytes, one batched unpack
+   launch; argument sets are kept for repeated calls. */
+int MPI_Neighbor_alltoallw(const void *sendbuf, const int sendcounts[],
+                           const MPI_Aint sdispls[],
+                           const MPI_Datatype sendtypes[], void *recvbuf,
+                           const int recvcounts[], const MPI_Aint rdispls[],
+                           const MPI_Datatype recvtypes[], MPI_Comm comm);
+/* reference: src/comm_free.cpp -- drops state kept for the communicator */
+int MPI_Comm_free(MPI_Comm *comm);
 
 /* ------------------------------------------------------------------------ */
 /* 2. tempi_* entry points                                                  */

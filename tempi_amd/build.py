@@ -32,6 +32,7 @@ SOURCES = [
     "perf.hip",
     "pool.cpp",
     "p2p.cpp",
+    "neighbor.cpp",
 ]
 
 ARCH = os.environ.get("TEMPI_ARCH", "gfx950")

@@ -110,6 +110,11 @@ struct RealMpi {
   int (*Wait)(MPI_Request *, MPI_Status *) = nullptr;
   int (*Waitall)(int, MPI_Request *, MPI_Status *) = nullptr;
   int (*Test)(MPI_Request *, int *, MPI_Status *) = nullptr;
+  int (*Neighbor_alltoallw)(const void *, const int *, const MPI_Aint *,
+                            const MPI_Datatype *, void *, const int *,
+                            const MPI_Aint *, const MPI_Datatype *,
+                            MPI_Comm) = nullptr;
+  int (*Comm_free)(MPI_Comm *) = nullptr;
 };
 // resolves on first call; aborts with a message if a symbol is missing
 const RealMpi &real_mpi();
@@ -249,6 +254,13 @@ void pool_trim();
 // point-to-point state (p2p.cpp)
 void p2p_init();     // at MPI_Init: which ranks share a node
 void p2p_finalize(); // at MPI_Finalize: completes or reports pending work
+// the staging method forced by tempi_set_datatype_method or TEMPI_DATATYPE_*
+// (DEVICE becomes STAGED without a GPU-aware library); UNKNOWN: by model
+int forced_datatype_method();
+// neighbour exchanges (neighbor.cpp): kept argument sets
+void neighbor_finalize();
+void neighbor_forget_type(MPI_Datatype t);
+void neighbor_forget_comm(MPI_Comm c);
 
 // exhaustive search; fills best and, when given, the best time per variant
 int autotune(const Geom &g, const char *strided, char *packed, int dir,

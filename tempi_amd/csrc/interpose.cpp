@@ -115,6 +115,7 @@ TEMPI_EXPORT int MPI_Finalize(void) {
       std::lock_guard<std::mutex> lk(c.mu);
       c.map.clear();
     }
+    neighbor_finalize();
     p2p_finalize();
     finalize_runtime();
   }
@@ -152,9 +153,12 @@ TEMPI_EXPORT int MPI_Type_free(MPI_Datatype *datatype) {
   if (!env().disable && datatype) {
     // the library may change the handle, and may reuse its value for a new
     // type afterwards: drop the record first (reference: type_free.cpp:20)
-    TypeCache &c = type_cache();
-    std::lock_guard<std::mutex> lk(c.mu);
-    c.map.erase(key_of(*datatype));
+    {
+      TypeCache &c = type_cache();
+      std::lock_guard<std::mutex> lk(c.mu);
+      c.map.erase(key_of(*datatype));
+    }
+    neighbor_forget_type(*datatype);
   }
   return real_mpi().Type_free(datatype);
 }

@@ -97,16 +97,7 @@ bool colocated(MPI_Comm comm, int peer) {
 std::map<std::tuple<bool, int64_t, int64_t>, int> g_method_cache;
 
 int choose_method(const Canon &cn, int64_t bytes, MPI_Comm comm, int peer) {
-  int m = g_override_method >= 0 ? g_override_method : env().datatype_method;
-  if (m == TEMPI_METHOD_DEVICE && !env().gpu_aware_mpi) {
-    static bool warned = false;
-    if (!warned) {
-      warned = true;
-      TEMPI_LOG(LOG_WARN, "DEVICE sends need a GPU-aware MPI library "
-                          "(TEMPI_MPI_GPU_AWARE); staging through the host");
-    }
-    m = TEMPI_METHOD_STAGED;
-  }
+  const int m = forced_datatype_method();
   if (m != TEMPI_METHOD_UNKNOWN) {
     return m;
   }
@@ -556,6 +547,20 @@ int check_count(int count, const Canon &cn, int64_t *bytes) {
 }
 
 } // namespace
+
+int forced_datatype_method() {
+  int m = g_override_method >= 0 ? g_override_method : env().datatype_method;
+  if (m == TEMPI_METHOD_DEVICE && !env().gpu_aware_mpi) {
+    static bool warned = false;
+    if (!warned) {
+      warned = true;
+      TEMPI_LOG(LOG_WARN, "DEVICE sends need a GPU-aware MPI library "
+                          "(TEMPI_MPI_GPU_AWARE); staging through the host");
+    }
+    m = TEMPI_METHOD_STAGED;
+  }
+  return m;
+}
 
 void p2p_init() {
   int size = 1;

@@ -115,6 +115,10 @@ class Mpi:
         t.MPI_Waitall.argtypes = [I, Rp, P]
         t.MPI_Test.argtypes = [Rp, Ip, P]
         t.MPI_Get_count.argtypes = [P, H, Ip]
+        t.MPI_Dist_graph_create_adjacent.argtypes = [H, I, Ip, P, I, Ip, P, H, I, Hp]
+        Ap = ctypes.POINTER(ctypes.c_ssize_t)
+        t.MPI_Neighbor_alltoallw.argtypes = [P, Ip, Ap, Hp, P, Ip, Ap, Hp, H]
+        t.MPI_Comm_free.argtypes = [Hp]
         self.status_size = self.const("sizeof(MPI_Status)")
         self.REQUEST_NULL = self.const("MPI_REQUEST_NULL")
 
@@ -313,6 +317,38 @@ class Mpi:
         raw = (ctypes.c_char * self.status_size)()
         err = self.tempi.MPI_Test(ctypes.byref(r), ctypes.byref(flag), raw)
         return err, flag.value, self._status_dict(raw, h), r.value
+
+    # -- neighbourhood collectives -------------------------------------------
+    def dist_graph(self, sources, destinations):
+        """MPI_Dist_graph_create_adjacent over MPI_COMM_WORLD (unweighted)."""
+        out = self.H()
+        src = (ctypes.c_int * max(len(sources), 1))(*sources)
+        dst = (ctypes.c_int * max(len(destinations), 1))(*destinations)
+        unweighted = ctypes.c_void_p(self.const("MPI_UNWEIGHTED"))
+        _ok(self.tempi.MPI_Dist_graph_create_adjacent(
+            self.COMM_WORLD, len(sources), src, unweighted, len(destinations), dst,
+            unweighted, self.const("MPI_INFO_NULL"), 0, ctypes.byref(out)),
+            "MPI_Dist_graph_create_adjacent")
+        return out
+
+    def comm_free(self, comm):
+        c = self.H(comm.value)
+        _ok(self.tempi.MPI_Comm_free(ctypes.byref(c)), "MPI_Comm_free")
+
+    def neighbor_alltoallw(self, sendbuf, sendcounts, sdispls, sendtypes, recvbuf,
+                           recvcounts, rdispls, recvtypes, comm):
+        def ints(xs):
+            return (ctypes.c_int * max(len(xs), 1))(*xs)
+
+        def aints(xs):
+            return (ctypes.c_ssize_t * max(len(xs), 1))(*xs)
+
+        def hs(xs):
+            return (self.H * max(len(xs), 1))(*[x.value for x in xs])
+
+        return self.tempi.MPI_Neighbor_alltoallw(
+            sendbuf, ints(sendcounts), aints(sdispls), hs(sendtypes), recvbuf,
+            ints(recvcounts), aints(rdispls), hs(recvtypes), comm)
 
     def strided_block(self, h):
         """(StridedBlock, size) of a type committed through the interposer, or

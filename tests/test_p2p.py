@@ -48,3 +48,26 @@ def test_method_override_arguments():
     assert L.tempi_set_datatype_method(99) == ta.TEMPI_ERR_ARG
     prev = L.tempi_set_datatype_method(2)  # ONESHOT
     assert L.tempi_set_datatype_method(prev) == 2
+
+
+def test_neighbor_alltoallw_host_buffers_go_to_the_library(mpi):
+    """Host buffers: the library's own MPI_Neighbor_alltoallw, on a graph
+    whose two edges lead from this process to itself."""
+    spec = ("vector", 16, 4, 9, tc.BYTE)
+    h = mpi.type_from_spec(spec)
+    o = orc.OracleType(spec)
+    comm = mpi.dist_graph([0, 0], [0, 0])
+    src = orc.splitmix_bytes(2 * o.extent(), 21)
+    dst = np.full(src.size, 0xAB, np.uint8)
+    before = ta.counters()["p2p_library"]
+    err = mpi.neighbor_alltoallw(src.ctypes.data, [1, 1], [0, o.extent()], [h, h],
+                                 dst.ctypes.data, [1, 1], [o.extent(), 0], [h, h], comm)
+    assert err == 0
+    assert ta.counters()["p2p_library"] == before + 1
+    # block i sent to the first neighbour lands at receive block i
+    expected = np.full(src.size, 0xAB, np.uint8)
+    o.unpack(o.pack(src, 0, 1), expected, o.extent(), 1)
+    o.unpack(o.pack(src, o.extent(), 1), expected, 0, 1)
+    assert (dst == expected).all()
+    mpi.comm_free(comm)
+    mpi.free(h)

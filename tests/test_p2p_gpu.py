@@ -172,3 +172,40 @@ def test_grid_faces_exchange_self(mpi):
     assert (dst.download() == expected).all()
     for h in handles:
         mpi.free(h)
+
+
+@pytest.mark.parametrize("method", [ONESHOT, STAGED])
+def test_neighbor_alltoallw_halo_exchange_self(mpi, method):
+    """A periodic halo exchange of one 134^3 float64 subdomain (radius 3)
+    whose 26 neighbours are all this process: the interior halo sent in
+    direction d lands in the exterior halo on the opposite side.  Called
+    twice with fresh data (the second call reuses the kept batches)."""
+    ta.lib().tempi_set_datatype_method(method)
+    dirs = tc.halo_directions()
+    lcr = (128, 128, 128)
+    stypes = [mpi.type_from_spec(tc.halo_type(3, lcr, d, 8)) for d in dirs]
+    rspecs = [tc.halo_type(3, lcr, tuple(-x for x in d), 8, exterior=True) for d in dirs]
+    rtypes = [mpi.type_from_spec(sp) for sp in rspecs]
+    comm = mpi.dist_graph([0] * len(dirs), [0] * len(dirs))
+    n = 134 ** 3 * 8
+    src = ta.DeviceBuffer(n)
+    dst = ta.DeviceBuffer(n)
+    c0 = ta.counters()[COUNTER[method]]
+    for rep in range(2):
+        host = orc.splitmix_bytes(n, 100 + rep)
+        src.upload(host)
+        dst.fill(0xAB)
+        expected = np.full(n, 0xAB, np.uint8)
+        for d, rs in zip(dirs, rspecs):
+            packed = orc.OracleType(tc.halo_type(3, lcr, d, 8)).pack(host, 0, 1)
+            orc.OracleType(rs).unpack(packed, expected, 0, 1)
+        k = len(dirs)
+        err = mpi.neighbor_alltoallw(src.addr(), [1] * k, [0] * k, stypes,
+                                     dst.addr(), [1] * k, [0] * k, rtypes, comm)
+        assert err == 0, rep
+        assert (dst.download() == expected).all(), rep
+    # one argument set built, then reused
+    assert ta.counters()[COUNTER[method]] == c0 + 1
+    mpi.comm_free(comm)
+    for h in stypes + rtypes:
+        mpi.free(h)
