@@ -215,6 +215,29 @@ def main():
             "hbm_gbs": round(2 * half / float(np.median(v)) / 1e3, 1)}
         for k, v in cold.items()}
 
+    # the launch floor: the same batch kernel moving one 64-B row, and the
+    # x face alone and the y + z faces alone, timed like the headline
+    floor = {}
+    tiny = ta.DeviceBuffer(64)
+    parts = {
+        "one_64B_row": [(ta.StridedBlock.make(0, 64, [64], [1]), grid.addr(), tiny.addr(), 1,
+                         ta.DIR_PACK)],
+        "x_face_only": [(faces[0]["sb"], grid.addr(), faces[0]["packed"].addr(), 1,
+                         ta.DIR_PACK)],
+        "y_z_faces_only": [(f["sb"], grid.addr(), f["packed"].addr(), 1, ta.DIR_PACK)
+                           for f in faces[1:]],
+    }
+    for name, reqs in parts.items():
+        b = ta.Batch(reqs)
+        for _ in range(args.warmup):
+            b.launch(stream)
+        _, dv = timed_loop(ta, lambda b=b: b.launch(stream), args.steps, stream)
+        floor[name] = round(dv * 1e6 / args.steps, 3)
+        b.destroy()
+    floor["headline_pack_unpack_average"] = round(batch_avg_us, 3)
+    extras["pack_batch_us_per_launch"] = floor
+    tiny.free()
+
     # synchronous MPI_Pack / MPI_Unpack calls (interposed), wall clock
     call_us = {}
     for f in faces:
