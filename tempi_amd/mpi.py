@@ -119,6 +119,7 @@ class Mpi:
         Ap = ctypes.POINTER(ctypes.c_ssize_t)
         t.MPI_Neighbor_alltoallw.argtypes = [P, Ip, Ap, Hp, P, Ip, Ap, Hp, H]
         t.MPI_Comm_free.argtypes = [Hp]
+        t.MPI_Error_class.argtypes = [I, Ip]
         self.status_size = self.const("sizeof(MPI_Status)")
         self.REQUEST_NULL = self.const("MPI_REQUEST_NULL")
 
@@ -317,6 +318,11 @@ class Mpi:
         raw = (ctypes.c_char * self.status_size)()
         err = self.tempi.MPI_Test(ctypes.byref(r), ctypes.byref(flag), raw)
         return err, flag.value, self._status_dict(raw, h), r.value
+
+    def error_class(self, code):
+        c = ctypes.c_int()
+        _ok(self.tempi.MPI_Error_class(code, ctypes.byref(c)), "MPI_Error_class")
+        return c.value
 
     # -- neighbourhood collectives -------------------------------------------
     def dist_graph(self, sources, destinations):

@@ -209,3 +209,37 @@ def test_neighbor_alltoallw_halo_exchange_self(mpi, method):
     mpi.comm_free(comm)
     for h in stypes + rtypes:
         mpi.free(h)
+
+
+def test_wildcards_proc_null_and_truncation(mpi):
+    """MPI_ANY_SOURCE / MPI_ANY_TAG receives report the real source and tag;
+    MPI_PROC_NULL goes to the library; a message longer than the receive is
+    a truncation error and leaves the receive buffer untouched."""
+    mpi.errors_return()
+    ta.lib().tempi_set_datatype_method(ONESHOT)
+    h, src, dst, expected = setup(mpi, FACE, 2, 19)
+    any_src, any_tag = mpi.const("MPI_ANY_SOURCE"), mpi.const("MPI_ANY_TAG")
+    proc_null = mpi.const("MPI_PROC_NULL")
+    one = np.full(dst.nbytes, 0xAB, np.uint8)
+    o = orc.OracleType(FACE)
+    host = orc.splitmix_bytes(dst.nbytes, 19)
+    o.unpack(o.pack(host, 0, 1), one, 0, 1)
+    err, rr = mpi.irecv(dst.addr(), 1, h, any_src, any_tag)
+    err, sr = mpi.isend(src.addr(), 1, h, 0, 77)
+    err, sts, _ = mpi.waitall([rr, sr], [h, h])
+    assert err == 0 and sts[0]["source"] == 0 and sts[0]["tag"] == 77
+    assert (dst.download() == one).all()
+    lib0 = ta.counters()["p2p_library"]
+    err, nr = mpi.isend(src.addr(), 1, h, proc_null, 78)
+    assert err == 0
+    err, _, nr = mpi.wait(nr)
+    assert err == 0 and ta.counters()["p2p_library"] == lib0 + 1
+    dst.fill(0xAB)
+    err, rr = mpi.irecv(dst.addr(), 1, h, 0, 79)
+    err, sr = mpi.isend(src.addr(), 2, h, 0, 79)
+    err, _, rr = mpi.wait(rr, h)
+    assert err != 0 and mpi.error_class(err) == mpi.ERR_TRUNCATE
+    err, _, sr = mpi.wait(sr)
+    assert err == 0
+    assert (dst.download() == 0xAB).all()
+    mpi.free(h)
