@@ -71,3 +71,52 @@ def test_neighbor_alltoallw_host_buffers_go_to_the_library(mpi):
     assert (dst == expected).all()
     mpi.comm_free(comm)
     mpi.free(h)
+
+
+def test_two_ranks_host_buffers(tmp_path):
+    """Two processes under mpiexec (CPU only): MPI_Init through the library
+    builds its node map with one MPI_Allgather, and host-buffer sends of a
+    derived type between the ranks go to the MPI library and arrive intact."""
+    import os
+    import shutil
+    import subprocess
+    import sys
+
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if not os.path.exists(mpiexec):
+        import pytest
+        pytest.skip("no mpiexec")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = tmp_path / "two.py"
+    script.write_text(
+        "import sys, ctypes\n"
+        f"sys.path.insert(0, {root!r}); sys.path.insert(0, {os.path.join(root, 'tests')!r})\n"
+        "import numpy as np\n"
+        "import tempi_amd as ta, typecatalog as tc\n"
+        "from tempi_amd import mpi as tmpi\n"
+        "from oracle import oracle as orc\n"
+        "m = tmpi.get()\n"
+        "rank = ctypes.c_int()\n"
+        "m.tempi.MPI_Comm_rank.argtypes = [m.H, ctypes.POINTER(ctypes.c_int)]\n"
+        "m.tempi.MPI_Comm_rank(m.COMM_WORLD, ctypes.byref(rank))\n"
+        "spec = ('vector', 32, 5, 11, tc.BYTE)\n"
+        "h = m.type_from_spec(spec)\n"
+        "o = orc.OracleType(spec)\n"
+        "src = orc.splitmix_bytes(o.extent(), 31)\n"
+        "if rank.value == 0:\n"
+        "    assert m.send(src.ctypes.data, 1, h, 1, 9) == 0\n"
+        "else:\n"
+        "    dst = np.full(o.extent(), 0xAB, np.uint8)\n"
+        "    err, st = m.recv(dst.ctypes.data, 1, h, 0, 9)\n"
+        "    exp = np.full(o.extent(), 0xAB, np.uint8)\n"
+        "    o.unpack(o.pack(src, 0, 1), exp, 0, 1)\n"
+        "    assert err == 0 and st['count'] == 1 and st['source'] == 0\n"
+        "    assert (dst == exp).all()\n"
+        "    print('rank 1 ok', flush=True)\n"
+        "m.free(h)\n"
+        "m.finalize()\n")
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="", ROCR_VISIBLE_DEVICES="")
+    r = subprocess.run([mpiexec, "-n", "2", "-hosts", "127.0.0.1", sys.executable, str(script)],
+                       capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "rank 1 ok" in r.stdout
