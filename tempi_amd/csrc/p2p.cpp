@@ -251,6 +251,10 @@ public:
   IsendOp(int64_t bytes, int peer, int tag, MPI_Comm comm)
       : bytes_(bytes), peer_(peer), tag_(tag), comm_(comm) {}
   ~IsendOp() override {
+    if (state_ == PACKING) {
+      // never completed (an error path): the pack may still be running
+      (void)hipStreamSynchronize(library_stream());
+    }
     if (ev_) {
       (void)hipEventDestroy(ev_);
     }
@@ -350,6 +354,9 @@ public:
   IrecvOp(const Canon &cn, int count, void *ubuf_dev, int64_t bytes)
       : cn_(cn), count_(count), ubuf_dev_(ubuf_dev), bytes_(bytes) {}
   ~IrecvOp() override {
+    if (state_ == UNPACKING) {
+      (void)hipStreamSynchronize(library_stream());
+    }
     if (ev_) {
       (void)hipEventDestroy(ev_);
     }
