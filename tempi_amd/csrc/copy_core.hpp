@@ -86,6 +86,38 @@ __device__ __forceinline__ void store_word(char *p, typename WordT<W>::T v) {
   }
 }
 
+// Copy n < 16 bytes to d from s (any alignment) with the widest stores d's
+// alignment allows: 1, 2, 4 then 8 bytes on the way up to an aligned
+// address, 8, 4, 2 then 1 on the way down from one (at most four stores
+// either way).
+__device__ __forceinline__ void copy_short(char *d, const char *s,
+                                           uint32_t n) {
+  while (n) {
+    const uintptr_t a = (uintptr_t)d;
+    if ((a & 1) || n < 2) {
+      *d = *s;
+      d += 1;
+      s += 1;
+      n -= 1;
+    } else if ((a & 2) || n < 4) {
+      store_word<2, false>(d, load_word<2, false>(s));
+      d += 2;
+      s += 2;
+      n -= 2;
+    } else if ((a & 4) || n < 8) {
+      store_word<4, false>(d, load_word<4, false>(s));
+      d += 4;
+      s += 4;
+      n -= 4;
+    } else {
+      store_word<8, false>(d, load_word<8, false>(s));
+      d += 8;
+      s += 8;
+      n -= 8;
+    }
+  }
+}
+
 // byte offset of row `row` on the strided side
 __device__ __forceinline__ int64_t row_offset(const KParams &p, uint32_t row) {
   int64_t off = 0;
@@ -142,11 +174,17 @@ __device__ __forceinline__ void load_unit(const KParams &p, uint32_t u,
     if (seg == 0) {
       const int64_t t0 = (int64_t)head + (int64_t)io.n * W;
       const uint32_t tail = (uint32_t)(p.L - t0);
-      for (uint32_t b = lane; b < head; b += P) {
-        d[b] = s[b];
-      }
-      for (uint32_t b = lane; b < tail; b += P) {
-        d[t0 + b] = s[t0 + b];
+      if constexpr (P == 1) {
+        // one lane per row: the widest aligned stores, at most four each
+        copy_short(d, s, head);
+        copy_short(d + t0, s + t0, tail);
+      } else {
+        for (uint32_t b = lane; b < head; b += P) {
+          d[b] = s[b];
+        }
+        for (uint32_t b = lane; b < tail; b += P) {
+          d[t0 + b] = s[t0 + b];
+        }
       }
     }
     s += head;
