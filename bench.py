@@ -80,8 +80,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share of the pack path; --gpus {args.gpus} is not "
-              "supported (the pack path does not communicate; DESIGN.md describes how it "
-              "shards).", file=sys.stderr)
+              "supported: packing does not communicate, so N GPUs run N independent copies "
+              "of this measurement, each on the subdomains it owns.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -1,7 +1,8 @@
 """Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
 
 * golden vectors recorded from MPICH's MPI_Pack / MPI_Unpack (no MPI needed);
-* known answers probed from MPICH (SURVEY-recorded probes);
+* known answers probed from MPICH 3.3.2 by hand (vector(3,2,4,BYTE), a
+  column-major hvector, a padded subarray, a count-1 subarray dimension);
 * closed forms: packed size, vector / subarray extents, subarray start;
 * the two oracle halves (typemap walker, strided-block gather) against each
   other, and against live MPICH on seeded random types when it is present;
