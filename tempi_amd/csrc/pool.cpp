@@ -6,8 +6,9 @@
 // Here: one free list per size class (powers of two from 4 KiB) for device
 // memory (hipMalloc) and one for pinned host memory mapped into the device
 // address space (hipHostMalloc, Mapped | Portable: the kernels store packed
-// bytes into it directly).  Buffers return to their list on release and are
-// freed at MPI_Finalize.
+// bytes into it directly).  Buffers return to their list on release (up to
+// 2 GiB of free buffers per kind; past that a released buffer is freed) and
+// the lists are freed at MPI_Finalize.
 #include "internal.hpp"
 
 #include <hip/hip_runtime.h>
@@ -23,7 +24,11 @@ struct Pool {
   std::mutex mu;
   std::multimap<size_t, void *> free_list; // size class -> buffer
   std::unordered_map<void *, size_t> size_of;
+  size_t free_bytes = 0; // held in free_list
 };
+
+// free buffers kept per kind; beyond this a released buffer is freed
+constexpr size_t kKeepBytes = 2ull << 30;
 
 Pool &pool(bool host) {
   static Pool device_pool, host_pool;
@@ -49,6 +54,7 @@ void *pool_acquire(bool host, size_t bytes) {
     if (it != p.free_list.end()) {
       void *b = it->second;
       p.free_list.erase(it);
+      p.free_bytes -= c;
       return b;
     }
   }
@@ -74,9 +80,16 @@ void pool_release(bool host, void *b) {
   Pool &p = pool(host);
   std::lock_guard<std::mutex> lk(p.mu);
   auto it = p.size_of.find(b);
-  if (it != p.size_of.end()) {
-    p.free_list.emplace(it->second, b);
+  if (it == p.size_of.end()) {
+    return;
   }
+  if (p.free_bytes + it->second > kKeepBytes) {
+    (void)(host ? hipHostFree(b) : hipFree(b));
+    p.size_of.erase(it);
+    return;
+  }
+  p.free_bytes += it->second;
+  p.free_list.emplace(it->second, b);
 }
 
 void pool_trim() {
@@ -88,6 +101,7 @@ void pool_trim() {
       p.size_of.erase(kv.second);
     }
     p.free_list.clear();
+    p.free_bytes = 0;
   }
 }
 
