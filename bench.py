@@ -234,6 +234,14 @@ def main():
         _, dv = timed_loop(ta, lambda b=b: b.launch(stream), args.steps, stream)
         floor[name] = round(dv * 1e6 / args.steps, 3)
         b.destroy()
+    # the single-copy kernel moving the same 64-B row: the launch floor
+    # without any batch bookkeeping
+    one = parts["one_64B_row"][0][0]
+    for _ in range(args.warmup):
+        ta.pack_sb(one, grid.addr(), 1, tiny.addr(), 64, 0, -1, stream)
+    _, dv = timed_loop(ta, lambda: ta.pack_sb(one, grid.addr(), 1, tiny.addr(), 64, 0, -1,
+                                              stream), args.steps, stream)
+    floor["one_64B_row_single_copy_kernel"] = round(dv * 1e6 / args.steps, 3)
     floor["headline_pack_unpack_average"] = round(batch_avg_us, 3)
     extras["pack_batch_us_per_launch"] = floor
     tiny.free()
@@ -298,7 +306,8 @@ def main():
         "selector": "model" if ta.lib().tempi_mlp_loaded() else "closed-form rule",
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_batch",
+            # batches of up to 8 requests launch k_batch_small
+            "kernel": "k_batch_small" if len(faces) <= 8 else "k_batch",
             "achieved": round(achieved_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

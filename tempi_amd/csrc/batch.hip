@@ -113,6 +113,56 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// A batch of at most kSmallItems requests travels whole in the kernel
+// arguments.  k_batch's workgroups wait on three dependent loads before
+// their first data load (XCD range, tile record, request parameters); here
+// the tile follows from the workgroup index by arithmetic on the argument
+// block, and the parameters come from the same block, which saves those
+// round trips on launches that last only a few microseconds.
+//
+// Padded tile numbering: request r owns tiles [base[r], base[r+1]), a range
+// that starts at a multiple of 8 and has 8 * share[r] entries.  Tile T runs
+// on XCD T % 8 (the grid is a multiple of 8, so T % 8 == blockIdx.x % 8 on
+// every pass), and XCD x takes the request's tiles
+// [x * share[r], (x + 1) * share[r]): each XCD gets a contiguous eighth of
+// every request, as in k_batch.  Padding entries past the request's last
+// tile do nothing.
+constexpr int kSmallItems = 8;
+
+struct SmallBatch {
+  uint32_t n;                     // requests
+  uint32_t end;                   // padded tiles of the whole batch
+  uint32_t base[kSmallItems];     // first padded tile of each request
+  uint32_t share[kSmallItems];    // tiles of each request per XCD
+  uint32_t ntiles[kSmallItems];   // tiles of each request
+  BatchItem items[kSmallItems];
+};
+
+// The argument is read in place through the kernel-argument segment pointer
+// (it is the only argument, at offset 0): indexing the by-value parameter
+// with a run-time request number would copy all of it to scratch memory.
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(5)))
+    k_batch_small(const SmallBatch arg) {
+  (void)arg;
+  const SmallBatch *sb =
+      (const SmallBatch *)__builtin_amdgcn_kernarg_segment_ptr();
+  const uint32_t end = sb->end;
+  for (uint32_t t = blockIdx.x; t < end; t += gridDim.x) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 1; i < kSmallItems; ++i) {
+      r += (i < (int)sb->n && t >= sb->base[i]) ? 1u : 0u;
+    }
+    const uint32_t q = t - sb->base[r];
+    const uint32_t l = (q % kXcds) * sb->share[r] + q / kXcds;
+    if (l < sb->ntiles[r]) {
+      const BatchItem &it = sb->items[r];
+      run_tile(it.kind, it.p, l * it.units_per_tile);
+    }
+  }
+}
+
 } // namespace
 
 } // namespace tempi
@@ -130,18 +180,28 @@ struct tempi_batch {
   Tile *d_tiles = nullptr;          // tile records, grouped by XCD
   uint32_t *d_xcd_begin = nullptr;  // first tile of each XCD (kXcds + 1)
   int xcd_split = 0;
+  bool small = false; // launched as k_batch_small with `sb`
+  SmallBatch sb{};
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<int> variants;
 };
 
+static void launch_kernel(const tempi_batch *b, hipStream_t s) {
+  if (b->small) {
+    hipLaunchKernelGGL(k_batch_small, dim3(b->grid), dim3(kBlock), 0, s, b->sb);
+  } else {
+    hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0, s, b->d_items,
+                       b->d_tiles, b->d_xcd_begin, b->ntiles, b->xcd_split);
+  }
+}
+
 static int batch_launch_on(tempi_batch *b, hipStream_t s) {
   if (b->ntiles == 0) {
     return TEMPI_SUCCESS;
   }
-  hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0, s, b->d_items,
-                     b->d_tiles, b->d_xcd_begin, b->ntiles, b->xcd_split);
+  launch_kernel(b, s);
   TEMPI_HIP_CHECK(hipGetLastError());
   return TEMPI_SUCCESS;
 }
@@ -227,6 +287,27 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   // persistent grid: up to 8 workgroups per CU, a multiple of the XCD count
   // when the tiles are split by XCD
   const int64_t cap = (int64_t)device_cu_count() * 8;
+  if (!items.empty() && items.size() <= (size_t)kSmallItems) {
+    // the whole batch rides in the kernel arguments; no device tables
+    SmallBatch &sb = b->sb;
+    sb.n = (uint32_t)items.size();
+    uint32_t base = 0;
+    for (size_t r = 0; r < items.size(); ++r) {
+      sb.base[r] = base;
+      sb.share[r] = (item_tiles[r] + kXcds - 1) / kXcds;
+      sb.ntiles[r] = item_tiles[r];
+      sb.items[r] = items[r];
+      base += kXcds * sb.share[r];
+    }
+    for (size_t r = items.size(); r < (size_t)kSmallItems; ++r) {
+      sb.base[r] = base;
+    }
+    sb.end = base;
+    b->small = true;
+    b->grid = (int)std::min<int64_t>((int64_t)base, cap / kXcds * kXcds);
+    *out = b;
+    return TEMPI_SUCCESS;
+  }
   if ((int64_t)tiles >= 8 * (int64_t)kXcds) {
     b->xcd_split = 1;
     b->grid = (int)(std::min<int64_t>((int64_t)tiles, cap) / kXcds * kXcds);
@@ -307,9 +388,7 @@ TEMPI_EXPORT int tempi_batch_graph_launch(tempi_batch *b, void *stream) {
         hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
     TEMPI_HIP_CHECK(hipStreamBeginCapture(b->capture_stream,
                                           hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0,
-                       b->capture_stream, b->d_items, b->d_tiles,
-                       b->d_xcd_begin, b->ntiles, b->xcd_split);
+    launch_kernel(b, b->capture_stream);
     TEMPI_HIP_CHECK(hipStreamEndCapture(b->capture_stream, &b->graph));
     TEMPI_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }

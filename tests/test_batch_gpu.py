@@ -191,3 +191,43 @@ def test_empty_and_zero_count_requests():
     b2 = ta.Batch([])
     b2.launch()
     ta.synchronize()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 9])
+def test_small_batches(k):
+    """Batches of up to 8 requests launch with the whole batch in the kernel
+    arguments; 9 takes the table-driven launch.  One request is large enough
+    that its tiles outnumber the persistent grid (several passes per
+    workgroup), the others are random shapes of both directions."""
+    rng = random.Random(500 + k)
+    big = ta.StridedBlock.make(4, 4160 * 16384, [4096, 16384], [1, 4160])
+    shapes = [(big, big.extent)] + [random_sb(rng) for _ in range(k - 1)]
+    span = sum(e + 64 for _, e in shapes)
+    host = orc.splitmix_bytes(span, 40 + k)
+    sizes = [sb.size() for sb, _ in shapes]
+    phost = orc.splitmix_bytes(sum(sizes), 80 + k)
+    src = ta.DeviceBuffer(span)
+    src.upload(host)
+    packed = ta.DeviceBuffer(sum(sizes))
+    packed.upload(phost)
+    reqs, meta, o, po = [], [], 0, 0
+    for i, ((sb, e), sz) in enumerate(zip(shapes, sizes)):
+        d = ta.DIR_UNPACK if i % 2 else ta.DIR_PACK
+        reqs.append((sb, src.addr(o), packed.addr(po), 1, d))
+        meta.append((sb, d, o, po, sz))
+        o += e + 64
+        po += sz
+    b = ta.Batch(reqs)
+    # more tiles than 8 workgroups on each of MI355X's 256 CUs
+    assert b.info()["tiles"] > 8 * 256
+    b.launch()
+    ta.synchronize()
+    got_s, got_p = src.download(), packed.download()
+    exp_s = host.copy()
+    for sb, d, so, pk, sz in meta:
+        desc = orc.sb_desc(*sb.as_tuple())
+        if d == ta.DIR_PACK:
+            assert (got_p[pk:pk + sz] == orc.sb_pack(desc, host, so, 1)).all(), (k, sb)
+        else:
+            orc.sb_unpack(desc, phost[pk:pk + sz], exp_s, so, 1)
+    assert (got_s == exp_s).all(), k
