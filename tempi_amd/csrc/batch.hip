@@ -113,12 +113,14 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// A batch of at most kSmallItems requests travels whole in the kernel
-// arguments.  k_batch's workgroups wait on three dependent loads before
-// their first data load (XCD range, tile record, request parameters); here
-// the tile follows from the workgroup index by arithmetic on the argument
-// block, and the parameters come from the same block, which saves those
-// round trips on launches that last only a few microseconds.
+// Batches of at most kSmallItems requests pass their tile index in the
+// kernel arguments.  k_batch's workgroups wait on three dependent loads
+// before their first data load (XCD range, tile record, request parameters);
+// here the tile follows from the workgroup index by arithmetic on the
+// argument block, leaving one dependent load (the request's parameters).
+// The parameters stay in device memory: a back-to-back launch with 2 KiB of
+// kernel arguments costs about 0.5 us more than one with a few hundred bytes
+// on MI355X, more than the loads it would save.
 //
 // Padded tile numbering: request r owns tiles [base[r], base[r+1]), a range
 // that starts at a multiple of 8 and has 8 * share[r] entries.  Tile T runs
@@ -129,35 +131,34 @@ __global__ void __launch_bounds__(kBlock)
 // tile do nothing.
 constexpr int kSmallItems = 8;
 
-struct SmallBatch {
+struct SmallIndex {
   uint32_t n;                     // requests
   uint32_t end;                   // padded tiles of the whole batch
   uint32_t base[kSmallItems];     // first padded tile of each request
   uint32_t share[kSmallItems];    // tiles of each request per XCD
   uint32_t ntiles[kSmallItems];   // tiles of each request
-  BatchItem items[kSmallItems];
 };
 
-// The argument is read in place through the kernel-argument segment pointer
-// (it is the only argument, at offset 0): indexing the by-value parameter
-// with a run-time request number would copy all of it to scratch memory.
+// The index is read in place through the kernel-argument segment pointer (it
+// is the first argument, at offset 0): indexing the by-value parameter with
+// a run-time request number would copy it to scratch memory.
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(5)))
-    k_batch_small(const SmallBatch arg) {
+    k_batch_small(const SmallIndex arg, const BatchItem *__restrict__ items) {
   (void)arg;
-  const SmallBatch *sb =
-      (const SmallBatch *)__builtin_amdgcn_kernarg_segment_ptr();
-  const uint32_t end = sb->end;
+  const SmallIndex *ix =
+      (const SmallIndex *)__builtin_amdgcn_kernarg_segment_ptr();
+  const uint32_t end = ix->end;
   for (uint32_t t = blockIdx.x; t < end; t += gridDim.x) {
     uint32_t r = 0;
 #pragma unroll
     for (int i = 1; i < kSmallItems; ++i) {
-      r += (i < (int)sb->n && t >= sb->base[i]) ? 1u : 0u;
+      r += (i < (int)ix->n && t >= ix->base[i]) ? 1u : 0u;
     }
-    const uint32_t q = t - sb->base[r];
-    const uint32_t l = (q % kXcds) * sb->share[r] + q / kXcds;
-    if (l < sb->ntiles[r]) {
-      const BatchItem &it = sb->items[r];
+    const uint32_t q = t - ix->base[r];
+    const uint32_t l = (q % kXcds) * ix->share[r] + q / kXcds;
+    if (l < ix->ntiles[r]) {
+      const BatchItem &it = items[r];
       run_tile(it.kind, it.p, l * it.units_per_tile);
     }
   }
@@ -180,8 +181,8 @@ struct tempi_batch {
   Tile *d_tiles = nullptr;          // tile records, grouped by XCD
   uint32_t *d_xcd_begin = nullptr;  // first tile of each XCD (kXcds + 1)
   int xcd_split = 0;
-  bool small = false; // launched as k_batch_small with `sb`
-  SmallBatch sb{};
+  bool small = false; // launched as k_batch_small with `ix`
+  SmallIndex ix{};
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -190,7 +191,8 @@ struct tempi_batch {
 
 static void launch_kernel(const tempi_batch *b, hipStream_t s) {
   if (b->small) {
-    hipLaunchKernelGGL(k_batch_small, dim3(b->grid), dim3(kBlock), 0, s, b->sb);
+    hipLaunchKernelGGL(k_batch_small, dim3(b->grid), dim3(kBlock), 0, s, b->ix,
+                       b->d_items);
   } else {
     hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0, s, b->d_items,
                        b->d_tiles, b->d_xcd_begin, b->ntiles, b->xcd_split);
@@ -288,27 +290,24 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   // when the tiles are split by XCD
   const int64_t cap = (int64_t)device_cu_count() * 8;
   if (!items.empty() && items.size() <= (size_t)kSmallItems) {
-    // the whole batch rides in the kernel arguments; no device tables
-    SmallBatch &sb = b->sb;
-    sb.n = (uint32_t)items.size();
+    // tile index in the kernel arguments; only the parameters go to the
+    // device table
+    SmallIndex &ix = b->ix;
+    ix.n = (uint32_t)items.size();
     uint32_t base = 0;
     for (size_t r = 0; r < items.size(); ++r) {
-      sb.base[r] = base;
-      sb.share[r] = (item_tiles[r] + kXcds - 1) / kXcds;
-      sb.ntiles[r] = item_tiles[r];
-      sb.items[r] = items[r];
-      base += kXcds * sb.share[r];
+      ix.base[r] = base;
+      ix.share[r] = (item_tiles[r] + kXcds - 1) / kXcds;
+      ix.ntiles[r] = item_tiles[r];
+      base += kXcds * ix.share[r];
     }
     for (size_t r = items.size(); r < (size_t)kSmallItems; ++r) {
-      sb.base[r] = base;
+      ix.base[r] = base;
     }
-    sb.end = base;
+    ix.end = base;
     b->small = true;
     b->grid = (int)std::min<int64_t>((int64_t)base, cap / kXcds * kXcds);
-    *out = b;
-    return TEMPI_SUCCESS;
-  }
-  if ((int64_t)tiles >= 8 * (int64_t)kXcds) {
+  } else if ((int64_t)tiles >= 8 * (int64_t)kXcds) {
     b->xcd_split = 1;
     b->grid = (int)(std::min<int64_t>((int64_t)tiles, cap) / kXcds * kXcds);
   } else {
@@ -316,9 +315,9 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   }
   // tile records grouped by XCD: XCD x takes the x-th eighth of every
   // item's tiles (a single group when the grid is not split)
-  const int groups = b->xcd_split ? (int)kXcds : 1;
+  const int groups = b->small ? 0 : b->xcd_split ? (int)kXcds : 1;
   std::vector<Tile> tile_list;
-  tile_list.reserve(tiles);
+  tile_list.reserve(b->small ? 0 : tiles);
   std::vector<uint32_t> xcd_begin(kXcds + 1, (uint32_t)tiles);
   for (int x = 0; x < groups; ++x) {
     xcd_begin[x] = (uint32_t)tile_list.size();
@@ -347,8 +346,10 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   std::vector<char> host(total, 0);
   if (!items.empty()) {
     memcpy(host.data(), items.data(), items_bytes);
-    memcpy(host.data() + tl_off, tile_list.data(),
-           tile_list.size() * sizeof(Tile));
+    if (!tile_list.empty()) {
+      memcpy(host.data() + tl_off, tile_list.data(),
+             tile_list.size() * sizeof(Tile));
+    }
   }
   memcpy(host.data() + xb_off, xcd_begin.data(), xcd_begin.size() * 4);
   // one upload, outside any capture

@@ -138,93 +138,87 @@ __device__ __forceinline__ int64_t row_offset(const KParams &p, uint32_t row) {
   return off;
 }
 
-// Loads of one work unit held in registers between the load and store
-// phases, so that a thread can have several units' loads in flight before
-// its first store.
-template <int W, int U> struct UnitIO {
-  typename WordT<W>::T v[U];
-  char *d;    // first body byte on the destination side
-  uint32_t w; // first word index of this lane
-  uint32_t n; // body words of the row
+// Addresses of one work unit, computed before any of the thread's loads.
+struct UnitAddr {
+  const char *s; // first body byte on the source side
+  char *d;       // first body byte on the destination side
+  uint32_t w;    // first word index of this lane
+  uint32_t n;    // body words of the row (0: nothing to move)
 };
 
-// Load phase of work unit `u` (wave-uniform when P == 64).  DIR 0 = pack
+// Address phase of work unit `u` (wave-uniform when P == 64).  DIR 0 = pack
 // (strided -> packed), 1 = unpack (packed -> strided).  Head and tail bytes
 // of a row whose destination is not word-aligned are copied here directly.
-template <int P, int W, int U, bool ALIGNED, int DIR, bool NTL>
-__device__ __forceinline__ void load_unit(const KParams &p, uint32_t u,
-                                          uint32_t lane, UnitIO<W, U> &io) {
+template <int P, int W, int U, bool ALIGNED, int DIR>
+__device__ __forceinline__ UnitAddr unit_addr(const KParams &p, uint32_t u,
+                                              uint32_t lane) {
   constexpr int A = word_align(W);
   const uint32_t row = p.nseg == 1 ? u : p.segdiv.div(u);
   const uint32_t seg = u - row * p.nseg;
   const int64_t soff = row_offset(p, row);
   const int64_t poff = (int64_t)row * p.L;
-  const char *s = p.src + (DIR == 0 ? soff : poff);
-  char *d = p.dst + (DIR == 0 ? poff : soff);
-  io.w = seg * (uint32_t)(P * U) + lane;
+  UnitAddr a;
+  a.s = p.src + (DIR == 0 ? soff : poff);
+  a.d = p.dst + (DIR == 0 ? poff : soff);
+  a.w = seg * (uint32_t)(P * U) + lane;
   if constexpr (ALIGNED) {
-    io.n = p.words;
+    a.n = p.words;
   } else {
-    const uint32_t mis = (uint32_t)((uintptr_t)d & (A - 1));
+    const uint32_t mis = (uint32_t)((uintptr_t)a.d & (A - 1));
     uint32_t head = mis ? (uint32_t)(A - mis) : 0u;
     if ((int64_t)head > p.L) {
       head = (uint32_t)p.L;
     }
-    io.n = (uint32_t)((p.L - head) / W);
+    a.n = (uint32_t)((p.L - head) / W);
     if (seg == 0) {
-      const int64_t t0 = (int64_t)head + (int64_t)io.n * W;
+      const int64_t t0 = (int64_t)head + (int64_t)a.n * W;
       const uint32_t tail = (uint32_t)(p.L - t0);
       if constexpr (P == 1) {
         // one lane per row: the widest aligned stores, at most four each
-        copy_short(d, s, head);
-        copy_short(d + t0, s + t0, tail);
+        copy_short(a.d, a.s, head);
+        copy_short(a.d + t0, a.s + t0, tail);
       } else {
         for (uint32_t b = lane; b < head; b += P) {
-          d[b] = s[b];
+          a.d[b] = a.s[b];
         }
         for (uint32_t b = lane; b < tail; b += P) {
-          d[t0 + b] = s[t0 + b];
+          a.d[t0 + b] = a.s[t0 + b];
         }
       }
     }
-    s += head;
-    d += head;
+    a.s += head;
+    a.d += head;
   }
-  io.d = d;
-#pragma unroll
-  for (int k = 0; k < U; ++k) {
-    const uint32_t w = io.w + k * P;
-    if (w < io.n) {
-      io.v[k] = load_word<W, NTL>(s + (size_t)w * W);
-    }
-  }
+  return a;
 }
 
-template <int P, int W, int U, bool NT>
-__device__ __forceinline__ void store_unit(const UnitIO<W, U> &io) {
-#pragma unroll
-  for (int k = 0; k < U; ++k) {
-    const uint32_t w = io.w + k * P;
-    if (w < io.n) {
-      store_word<W, NT>(io.d + (size_t)w * W, io.v[k]);
-    }
-  }
-}
+// s_waitcnt immediate for vmcnt(0) alone on gfx9 (expcnt and lgkmcnt at
+// their maxima, 7 and 15, so only the vector memory counter is waited on)
+constexpr int kWaitLoads = 0x0F70;
+
+// The words of one unit, held in registers between the load and store phases.
+template <int W, int U> struct UnitWords {
+  typename WordT<W>::T v[U];
+};
 
 // units a thread keeps in flight: enough for four loads per lane
 __host__ __device__ constexpr int units_in_flight(int U) {
   return U >= 4 ? 1 : 4 / U;
 }
 
-// Move units first, first + stride, ... (K of them, those below `end`):
-// all loads, then all stores.  NTS / NTL: nontemporal stores / loads of the
-// body words.
+// Move units first, first + stride, ... (K of them, those below `end`) in
+// three phases: every unit's addresses, then every load, then every store.
+// Keeping the loads in one straight run matters: with the address arithmetic
+// of later units between them, the compiler put a wait for the earlier loads
+// in front of the later ones (register reuse across the grid-stride loop),
+// and each unit paid a full memory round trip of its own.  NTS / NTL:
+// nontemporal stores / loads of the body words.
 template <int P, int W, int U, bool ALIGNED, int DIR, bool NTS, bool NTL>
 __device__ __forceinline__ void copy_units(const KParams &p, uint32_t first,
                                            uint32_t stride, uint32_t end,
                                            uint32_t lane) {
   constexpr int K = units_in_flight(U);
-  UnitIO<W, U> io[K];
+  UnitAddr a[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     uint32_t u = first + k * stride;
@@ -234,13 +228,39 @@ __device__ __forceinline__ void copy_units(const KParams &p, uint32_t first,
       u = __builtin_amdgcn_readfirstlane(u);
     }
     if (u < end) {
-      load_unit<P, W, U, ALIGNED, DIR, NTL>(p, u, lane, io[k]);
+      a[k] = unit_addr<P, W, U, ALIGNED, DIR>(p, u, lane);
+    } else {
+      a[k].s = nullptr;
+      a[k].d = nullptr;
+      a[k].w = 0;
+      a[k].n = 0;
     }
   }
+  UnitWords<W, U> v[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if (first + k * stride < end) {
-      store_unit<P, W, U, NTS>(io[k]);
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t w = a[k].w + j * P;
+      if (w < a[k].n) {
+        v[k].v[j] = load_word<W, NTL>(a[k].s + (size_t)w * W);
+      }
+    }
+  }
+  // Every load lands before any store, by one wait in straight-line code.
+  // Left to the compiler, the waits sit inside the per-word store
+  // conditions; on a path that skips a store its load counts as still
+  // pending at the loop back edge, and the next iteration waits for it in
+  // the middle of its own loads.
+  __builtin_amdgcn_s_waitcnt(kWaitLoads);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t w = a[k].w + j * P;
+      if (w < a[k].n) {
+        store_word<W, NTS>(a[k].d + (size_t)w * W, v[k].v[j]);
+      }
     }
   }
 }
