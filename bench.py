@@ -74,9 +74,9 @@ def main():
                     help="with --profile-only: also run one plain 256 MiB contiguous copy "
                          "(the reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--profile-large-blocks", action="store_true",
-                    help="only the 256 MiB 2D packs/unpacks (64 B - 4 KiB blocks), one "
-                         "launch each after a cache sweep, then the 256 MiB contiguous "
-                         "calibration copy (for rocprofv3)")
+                    help="only the 256 MiB 2D packs/unpacks (64 B - 4 KiB blocks, aligned "
+                         "and 4 B off), three cold launches each, then the 256 MiB "
+                         "contiguous calibration copy (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share of the pack path; --gpus {args.gpus} is not "
@@ -273,7 +273,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic_grid_faces_518.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get("k_batch_per_launch")
+            traffic = json.load(fh).get("per_launch")
 
     cpu = cpu_lib = None
     if not args.no_cpu:
@@ -328,21 +328,24 @@ def main():
     m.finalize()
 
 
-def run_default(ta, sb, d, src, dst, n, stream):
-    """One pack or unpack with the library's default choice, synchronized."""
+def run_default(ta, sb, d, src, dst, n, stream, off=0):
+    """One pack or unpack with the library's default choice, synchronized;
+    the strided side starts `off` bytes into `src`."""
     if d == ta.DIR_PACK:
-        ta.pack_sb(sb, src.addr(), 1, dst.addr(), n, 0, -1, stream)
+        ta.pack_sb(sb, src.addr(off), 1, dst.addr(), n, 0, -1, stream)
     else:
-        ta.unpack_sb(sb, dst.addr(), n, src.addr(), 1, 0, -1, stream)
+        ta.unpack_sb(sb, dst.addr(), n, src.addr(off), 1, 0, -1, stream)
     ta.synchronize(stream)
 
 
-def profile_large_blocks(ta, stream):
-    """One cold launch of each large-block copy with the library's default
+def profile_large_blocks(ta, stream, reps=3):
+    """Cold launches of each large-block copy with the library's default
     choice (settled by a first call of each shape, before the measured
-    launches), in a fixed order, then one 256 MiB contiguous copy (variant
-    1): the last nine strided-copy dispatches are the ones a counter run
-    attributes traffic to."""
+    launches): `reps` rounds over the copies in a fixed order -- rows of
+    64 B - 4 KiB, the strided side at a 64-B boundary and 4 bytes past one
+    (_off4) -- then one 256 MiB contiguous copy (variant 1).  The strided-copy
+    dispatches after the settling calls are the ones a trace or counter run
+    attributes to the copies printed here, in the same order."""
     n = 256 << 20
     src = ta.DeviceBuffer(3 << 30)
     src.fill(3)
@@ -353,22 +356,24 @@ def profile_large_blocks(ta, stream):
         rows = n // bl
         stride = 2 * bl + 64
         sb = ta.StridedBlock.make(0, rows * stride, [bl, rows], [1, stride])
-        for d, name in ((ta.DIR_PACK, "pack"), (ta.DIR_UNPACK, "unpack")):
-            run_default(ta, sb, d, src, dst, n, stream)  # settles the default choice
-            shapes.append((bl, sb, d, name))
-    for bl, sb, d, name in shapes:
-        v = ta.select_variant(sb, 1, src.addr(), dst.addr(), d, ta.SELECT_DEFAULT)
-        ta.flush_caches()
-        ev = [ta.Event(), ta.Event()]
-        ev[0].record(stream)
-        if d == ta.DIR_PACK:
-            ta.pack_sb(sb, src.addr(), 1, dst.addr(), n, 0, v, stream)
-        else:
-            ta.unpack_sb(sb, dst.addr(), n, src.addr(), 1, 0, v, stream)
-        ev[1].record(stream)
-        ta.synchronize(stream)
-        print(json.dumps({"copy": f"{name}_bl{bl}", "variant": v,
-                          "us": round(ev[0].elapsed_ms(ev[1]) * 1e3, 2)}), flush=True)
+        for off in (0, 4):
+            for d, name in ((ta.DIR_PACK, "pack"), (ta.DIR_UNPACK, "unpack")):
+                run_default(ta, sb, d, src, dst, n, stream, off)  # settles the choice
+                shapes.append((f"{name}_bl{bl}" + ("_off4" if off else ""), sb, d, off))
+    for rep in range(reps):
+        for name, sb, d, off in shapes:
+            v = ta.select_variant(sb, 1, src.addr(off), dst.addr(), d, ta.SELECT_DEFAULT)
+            ta.flush_caches()
+            ev = [ta.Event(), ta.Event()]
+            ev[0].record(stream)
+            if d == ta.DIR_PACK:
+                ta.pack_sb(sb, src.addr(off), 1, dst.addr(), n, 0, v, stream)
+            else:
+                ta.unpack_sb(sb, dst.addr(), n, src.addr(off), 1, 0, v, stream)
+            ev[1].record(stream)
+            ta.synchronize(stream)
+            print(json.dumps({"copy": name, "rep": rep, "variant": v,
+                              "us": round(ev[0].elapsed_ms(ev[1]) * 1e3, 2)}), flush=True)
     src.free()
     cal = ta.DeviceBuffer(n)
     cal.fill(1)
@@ -376,7 +381,7 @@ def profile_large_blocks(ta, stream):
     ta.flush_caches()
     ta.pack_sb(sb1, cal.addr(), 1, dst.addr(), n, 0, 1, stream)
     ta.synchronize(stream)
-    print(json.dumps({"copy": "contiguous_256MiB", "variant": 1}), flush=True)
+    print(json.dumps({"copy": "contiguous_256MiB", "rep": 0, "variant": 1}), flush=True)
 
 
 def bench_large_blocks(ta, stream):
@@ -384,7 +389,9 @@ def bench_large_blocks(ta, stream):
     = 2 x block + 64 B), each launch timed after a read-only cache sweep
     (the library times copies this large that way), with the library's
     default choice (what MPI_Pack runs) and with the autotuner's (variant,
-    grid cap)."""
+    grid cap).  Keys ending in _off4 repeat the copies with the strided
+    side 4 bytes past a 64-B boundary (float data at an odd element
+    offset): every row then starts and ends inside a cache line."""
     n = 256 << 20
     src = ta.DeviceBuffer(3 << 30)
     src.fill(3)
@@ -395,15 +402,16 @@ def bench_large_blocks(ta, stream):
         rows = n // bl
         stride = 2 * bl + 64
         sb = ta.StridedBlock.make(0, rows * stride, [bl, rows], [1, stride])
-        for d, name in ((ta.DIR_PACK, "pack"), (ta.DIR_UNPACK, "unpack")):
+        for (d, name), off in [(dn, o) for o in (0, 4) for dn in
+                               ((ta.DIR_PACK, "pack"), (ta.DIR_UNPACK, "unpack"))]:
             # the first call settles the default choice for the shape
-            run_default(ta, sb, d, src, dst, n, stream)
-            vd = ta.select_variant(sb, 1, src.addr(), dst.addr(), d, ta.SELECT_DEFAULT)
-            us_d = ta.time_variant(sb, 1, src.addr(), dst.addr(), d, vd, 0, 5)
-            va, ca, _ = ta.autotune(sb, 1, src.addr(), dst.addr(), d)
-            us_a = ta.time_variant(sb, 1, src.addr(), dst.addr(), d, va, ca, 5)
+            run_default(ta, sb, d, src, dst, n, stream, off)
+            vd = ta.select_variant(sb, 1, src.addr(off), dst.addr(), d, ta.SELECT_DEFAULT)
+            us_d = ta.time_variant(sb, 1, src.addr(off), dst.addr(), d, vd, 0, 5)
+            va, ca, _ = ta.autotune(sb, 1, src.addr(off), dst.addr(), d)
+            us_a = ta.time_variant(sb, 1, src.addr(off), dst.addr(), d, va, ca, 5)
             gd, ga = 2 * n / us_d / 1e3, 2 * n / us_a / 1e3
-            res[f"{name}_bl{bl}"] = {
+            res[f"{name}_bl{bl}" + ("_off4" if off else "")] = {
                 "default": {"variant": vd, "us": round(us_d, 2), "hbm_gbs": round(gd, 1),
                             "frac": round(gd / HBM_PEAK_GBS, 4)},
                 "autotuned": {"variant": va, "grid_cap": ca, "us": round(us_a, 2),

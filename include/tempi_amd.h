@@ -227,7 +227,8 @@ int tempi_launch_latency_us(int reps, float *host_us, float *device_us);
    A batch is a fixed set of pack and/or unpack requests (for example the 26
    faces, edges and corners of a 3D halo), resolved once into a
    device-resident descriptor table; one launch runs a persistent kernel that
-   drains all of them.  tempi_batch_graph_launch replays the same launch from
+   drains all of them (a batch of at most 8 requests passes its tile index
+   in the kernel arguments instead of a device tile table).  tempi_batch_graph_launch replays the same launch from
    a captured hipGraph.  Requests in one batch must be independent (no
    request writes bytes another reads or writes).  Nothing is synchronized:
    wait on the stream.  `method` (TEMPI_SELECT_*) picks the variant of each

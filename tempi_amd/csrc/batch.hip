@@ -33,6 +33,7 @@ struct BatchItem {
   KParams p;
   int32_t kind;            // 4 * variant + 2 * aligned + direction
   uint32_t units_per_tile; // units of one block-pass: groups per block * K
+  uint32_t ntiles;         // tiles of this request
 };
 
 constexpr uint32_t kXcds = 8;
@@ -40,29 +41,37 @@ constexpr uint32_t kXcds = 8;
 // requests smaller than this use the closed-form variant rule in a batch
 constexpr int64_t kBatchSelectMinBytes = 1ll << 20;
 
-// one block-pass over units [u0, u0 + GROUPS*K) of a request
+// One block-pass: tile `l` of a request split into `nt` tiles moves K
+// blocks of GROUPS units, nt * GROUPS units apart -- units l*GROUPS + g +
+// k*nt*GROUPS, every unit of the request exactly once over the nt tiles.
+// Spreading a lane's K units across the request rather than over K
+// neighbouring blocks measured about 0.2 us faster per launch on MI355X for
+// 12-B rows 2 KiB apart (262 144 rows), the same as the single-copy kernels'
+// grid-stride order.
 template <int P, int W, int U, bool ALIGNED, int DIR, bool NTS, bool NTL>
-__device__ __forceinline__ void copy_tile(const KParams &p, uint32_t u0) {
+__device__ __forceinline__ void copy_tile(const KParams &p, uint32_t l,
+                                          uint32_t nt) {
   constexpr uint32_t GROUPS = kBlock / P;
-  copy_units<P, W, U, ALIGNED, DIR, NTS, NTL>(p, u0 + threadIdx.x / P, GROUPS,
-                                              p.units, threadIdx.x % P);
+  copy_units<P, W, U, ALIGNED, DIR, NTS, NTL>(p, l * GROUPS + threadIdx.x / P,
+                                              nt * GROUPS, p.units,
+                                              threadIdx.x % P);
 }
 
 __device__ __forceinline__ void run_tile(int kind, const KParams &p,
-                                         uint32_t u0) {
+                                         uint32_t l, uint32_t nt) {
   switch (kind) {
 #define TEMPI_BCASE(I, P, W, U, NTS, NTL)                                      \
   case 4 * I + 0:                                                              \
-    copy_tile<P, W, U, false, 0, NTS, NTL>(p, u0);                             \
+    copy_tile<P, W, U, false, 0, NTS, NTL>(p, l, nt);                          \
     break;                                                                     \
   case 4 * I + 1:                                                              \
-    copy_tile<P, W, U, false, 1, NTS, NTL>(p, u0);                             \
+    copy_tile<P, W, U, false, 1, NTS, NTL>(p, l, nt);                          \
     break;                                                                     \
   case 4 * I + 2:                                                              \
-    copy_tile<P, W, U, true, 0, NTS, NTL>(p, u0);                              \
+    copy_tile<P, W, U, true, 0, NTS, NTL>(p, l, nt);                           \
     break;                                                                     \
   case 4 * I + 3:                                                              \
-    copy_tile<P, W, U, true, 1, NTS, NTL>(p, u0);                              \
+    copy_tile<P, W, U, true, 1, NTS, NTL>(p, l, nt);                           \
     break;
     TEMPI_VARIANTS(TEMPI_BCASE)
 #undef TEMPI_BCASE
@@ -71,10 +80,10 @@ __device__ __forceinline__ void run_tile(int kind, const KParams &p,
   }
 }
 
-// One tile: the request it belongs to and its first unit.
+// One tile: the request it belongs to and its index within the request.
 struct Tile {
   uint32_t item;
-  uint32_t u0;
+  uint32_t tile;
 };
 
 // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
@@ -107,9 +116,9 @@ __global__ void __launch_bounds__(kBlock)
     // would live in scratch memory)
     const Tile tl = tiles[t];
     const uint32_t r = __builtin_amdgcn_readfirstlane(tl.item);
-    const uint32_t u0 = __builtin_amdgcn_readfirstlane(tl.u0);
+    const uint32_t l = __builtin_amdgcn_readfirstlane(tl.tile);
     const BatchItem &it = items[r];
-    run_tile(it.kind, it.p, u0);
+    run_tile(it.kind, it.p, l, it.ntiles);
   }
 }
 
@@ -159,7 +168,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t l = (q % kXcds) * ix->share[r] + q / kXcds;
     if (l < ix->ntiles[r]) {
       const BatchItem &it = items[r];
-      run_tile(it.kind, it.p, l * it.units_per_tile);
+      run_tile(it.kind, it.p, l, it.ntiles);
     }
   }
 }
@@ -280,6 +289,7 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
       delete b;
       return set_error(TEMPI_ERR_UNSUPPORTED, "batch exceeds 2^31 tiles");
     }
+    it.ntiles = (uint32_t)t;
     item_tiles.push_back((uint32_t)t);
     items.push_back(it);
     b->bytes += g.L * g.rows;
@@ -326,7 +336,7 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
       const uint64_t lo = n_r * x / groups, hi = n_r * (x + 1) / groups;
       for (uint64_t l = lo; l < hi; ++l) {
         tile_list.push_back(
-            Tile{(uint32_t)r, (uint32_t)(l * items[r].units_per_tile)});
+            Tile{(uint32_t)r, (uint32_t)l});
       }
     }
   }
