@@ -243,3 +243,120 @@ def test_wildcards_proc_null_and_truncation(mpi):
     assert err == 0
     assert (dst.download() == 0xAB).all()
     mpi.free(h)
+
+
+RANK_SCRIPT = r'''
+import ctypes, sys
+sys.path.insert(0, ROOT); sys.path.insert(0, ROOT + "/tests")
+import numpy as np
+import tempi_amd as ta, typecatalog as tc
+from tempi_amd import mpi as tmpi
+from oracle import oracle as orc
+
+m = tmpi.get()
+rank = ctypes.c_int()
+m.tempi.MPI_Comm_rank.argtypes = [m.H, ctypes.POINTER(ctypes.c_int)]
+m.tempi.MPI_Comm_rank(m.COMM_WORLD, ctypes.byref(rank))
+me = rank.value
+peer = 1 - me
+print(f"rank {me} up", flush=True)
+
+
+def grid(seed, n):
+    host = orc.splitmix_bytes(n, seed)
+    d = ta.DeviceBuffer(n)
+    d.upload(host)
+    return host, d
+
+
+# each rank sends one type and receives the other's: the x face of a 38^3
+# float grid (12-B rows) one way, a byte vector the other way
+specs = [tc.grid_face_subarray(32, 3, "x"), ("vector", 300, 40, 77, tc.BYTE)]
+for method in (2, 3):  # ONESHOT, STAGED
+    ta.lib().tempi_set_datatype_method(method)
+    mine, theirs = specs[me], specs[peer]
+    ho = orc.OracleType(mine)
+    hs = m.type_from_spec(mine)
+    hr = m.type_from_spec(theirs)
+    tr = orc.OracleType(theirs)
+    src_host, src = grid(100 * method + me, ho.extent() * 2)
+    dst = ta.DeviceBuffer(tr.extent() * 2)
+    dst.fill(0xAB)
+    c0 = ta.counters()
+    err, rr = m.irecv(dst.addr(), 2, hr, peer, 7)
+    assert err == 0
+    err, sr = m.isend(src.addr(), 2, hs, peer, 7)
+    assert err == 0
+    err, sts, after = m.waitall([rr, sr], [hr, hs])
+    assert err == 0 and sts[0]["count"] == 2 and sts[0]["source"] == peer, sts
+    peer_host = orc.splitmix_bytes(tr.extent() * 2, 100 * method + peer)
+    exp = np.full(tr.extent() * 2, 0xAB, np.uint8)
+    tr.unpack(tr.pack(peer_host, 0, 2), exp, 0, 2)
+    assert (dst.download() == exp).all(), ("p2p", method)
+    key = {2: "p2p_oneshot", 3: "p2p_staged"}[method]
+    assert ta.counters()[key] == c0[key] + 2, (key, ta.counters())
+    m.free(hs)
+    m.free(hr)
+    print(f"rank {me} p2p method {method} ok", flush=True)
+
+# neighbourhood exchange over a 2-rank graph: the +x interior halo of a 20^3
+# float64 subdomain (radius 3) goes to the peer, whose -x exterior halo takes it
+lcr = (14, 14, 14)
+n = 20 ** 3 * 8
+comm = m.dist_graph([peer], [peer])
+st = m.type_from_spec(tc.halo_type(3, lcr, (1, 0, 0), 8))
+rspec = tc.halo_type(3, lcr, (-1, 0, 0), 8, exterior=True)
+rt = m.type_from_spec(rspec)
+for method in (2, 3):
+    ta.lib().tempi_set_datatype_method(method)
+    src_host, src = grid(500 + 10 * method + me, n)
+    dst = ta.DeviceBuffer(n)
+    dst.fill(0xAB)
+    err = m.neighbor_alltoallw(src.addr(), [1], [0], [st], dst.addr(), [1], [0], [rt], comm)
+    assert err == 0, err
+    peer_host = orc.splitmix_bytes(n, 500 + 10 * method + peer)
+    exp = np.full(n, 0xAB, np.uint8)
+    packed = orc.OracleType(tc.halo_type(3, lcr, (1, 0, 0), 8)).pack(peer_host, 0, 1)
+    orc.OracleType(rspec).unpack(packed, exp, 0, 1)
+    assert (dst.download() == exp).all(), ("neighbor", method)
+    print(f"rank {me} neighbor method {method} ok", flush=True)
+m.comm_free(comm)
+m.free(st)
+m.free(rt)
+ta.lib().tempi_set_datatype_method(-1)
+m.finalize()
+print(f"rank {me} done", flush=True)
+'''
+
+
+def test_two_ranks_device_buffers(tmp_path):
+    """Two processes on the one GPU under mpiexec: device-resident derived
+    types exchanged both ways through the interposed MPI_Isend / MPI_Irecv
+    (ONESHOT and STAGED) and MPI_Neighbor_alltoallw on a 2-rank graph; each
+    receiver checks the bytes against the oracle, gaps untouched."""
+    import os
+    import shutil
+    import signal
+    import subprocess
+    import sys
+
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if not os.path.exists(mpiexec):
+        pytest.skip("no mpiexec")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = tmp_path / "ranks.py"
+    script.write_text(f"ROOT = {root!r}\n" + RANK_SCRIPT)
+    print("two ranks: p2p ONESHOT/STAGED, neighbor_alltoallw", flush=True)
+    p = subprocess.Popen([mpiexec, "-n", "2", "-hosts", "127.0.0.1", sys.executable, str(script)],
+                         stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                         start_new_session=True)
+    try:
+        out, _ = p.communicate(timeout=120)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        out, _ = p.communicate()
+        raise AssertionError("two-rank exchange timed out:\n" + out)
+    print(out, flush=True)
+    assert p.returncode == 0, out
+    for r in (0, 1):
+        assert f"rank {r} done" in out, out
