@@ -152,6 +152,13 @@ def main():
         bit_exact &= bool((f["packed"].download() == expected[f["face"]]).all())
 
     extras = {}
+    # the variant of each face in the two batches (settled when the batches
+    # were created: the library times its candidates on the batch itself)
+    extras["batch_variants"] = {
+        "pack": [pack_batch.variant(i) for i in range(len(faces))],
+        "unpack": [unpack_batch.variant(i) for i in range(len(faces))],
+    }
+    log(f"batch variants (x, y, z faces): {extras['batch_variants']}")
     # -- the same step as six single-copy kernels (what MPI_Pack launches)
     def step_per_call():
         for f in faces:

@@ -174,6 +174,7 @@ int tempi_variant_info_get(int variant, tempi_variant_info *info);
 /* Variant the library would use for this copy.  Under the default policy a
    copy of 64 MiB or more is settled on its first run: the model's three
    best variants and the closed-form rule's pick are each timed once, cold,
+   writing a scratch image of the destination (never the caller's buffers),
    and the fastest is kept for the shape; before that run this returns the
    model's pick. */
 int tempi_select_variant(const tempi_strided_block *sb, int64_t incount,
@@ -334,6 +335,7 @@ typedef struct tempi_counters {
   int64_t p2p_oneshot;
   int64_t p2p_staged;
   int64_t p2p_library;          /* point-to-point calls forwarded as they are */
+  int64_t trial_launches;       /* copies the library timed (not the caller's) */
 } tempi_counters;
 int tempi_counters_get(tempi_counters *out);
 int tempi_counters_reset(void);

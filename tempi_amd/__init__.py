@@ -111,6 +111,7 @@ class Counters(ctypes.Structure):
             "p2p_oneshot",
             "p2p_staged",
             "p2p_library",
+            "trial_launches",
         )
     ]
 

@@ -28,7 +28,7 @@ const int kGridCaps[] = {0, 2, 4, 8, 32};
 // between real calls, and back-to-back repeats would rank the variants by
 // how they cope with the previous repeat's dirty lines instead.
 constexpr int64_t kColdBytes = 64ll << 20;
-constexpr int64_t kColdFlushBytes = 512ll << 20;
+constexpr int64_t kColdFlushBytes = kColdFlushBytesAll;
 const int kNumGridCaps = sizeof(kGridCaps) / sizeof(kGridCaps[0]);
 
 // Reads one dword per 64 B of `buf` and writes one word per workgroup: the
@@ -110,10 +110,12 @@ std::string shape_key_of(const Geom &g, const char *strided,
 }
 
 bool times_cold(const Geom &g) { return g.L * g.rows >= kColdBytes; }
+bool times_cold_bytes(int64_t bytes) { return bytes >= kColdBytes; }
 
 int time_config(const Geom &g, const char *strided, char *packed, int dir,
                 int variant, int max_blocks, int reps, hipStream_t stream,
                 float *us) {
+  TrialScope trial;
   const bool cold = times_cold(g);
   hipEvent_t a = nullptr, b = nullptr;
   TEMPI_HIP_CHECK(hipEventCreate(&a));
@@ -187,12 +189,91 @@ int autotune(const Geom &g, const char *strided, char *packed, int dir,
   return TEMPI_SUCCESS;
 }
 
+// Trial copies of the library's own choices (verification and the
+// autotune policy) write a scratch image of the destination, never the
+// caller's buffers: a batch resolves its variants when it is created, long
+// before it runs, and an unpack trial would otherwise overwrite the
+// caller's strided buffer with whatever the packed buffer held then.  The
+// image has the written side's length and its address modulo 256, so the
+// trials see the same alignment (and the same row, line and page pattern)
+// as the real copy; the read side is the caller's, untouched.
+TrialBuffers &TrialBuffers::operator=(TrialBuffers &&o) noexcept {
+  if (this != &o) {
+    this->~TrialBuffers();
+    mem = o.mem;
+    stream = o.stream;
+    strided = o.strided;
+    packed = o.packed;
+    o.mem = nullptr;
+  }
+  return *this;
+}
+
+TrialBuffers::~TrialBuffers() {
+  if (mem) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(mem);
+    mem = nullptr;
+  }
+}
+
+int64_t written_span(const Geom &g, int dir) {
+  if (dir != TEMPI_DIR_UNPACK) {
+    return g.rows * g.L;
+  }
+  int64_t lo = 0, hi = g.L;
+  for (int k = 0; k < g.nrd; ++k) {
+    const int64_t reach = (g.cnt[k] - 1) * g.str[k];
+    (reach < 0 ? lo : hi) += reach;
+  }
+  return hi - lo;
+}
+
+int trial_buffers(const Geom &g, const char *strided, char *packed, int dir,
+                  hipStream_t stream, TrialBuffers *t) {
+  // bytes the copy writes: the packed stream, or the strided rows' span
+  int64_t lo = 0, hi = g.L;
+  const char *first = packed;
+  if (dir == TEMPI_DIR_UNPACK) {
+    for (int k = 0; k < g.nrd; ++k) {
+      const int64_t reach = (g.cnt[k] - 1) * g.str[k];
+      (reach < 0 ? lo : hi) += reach;
+    }
+    first = strided + g.start + lo;
+  } else {
+    hi = g.rows * g.L;
+  }
+  const size_t bytes = (size_t)(hi - lo) + 256;
+  t->stream = stream;
+  if (hipSuccess != hipMalloc(&t->mem, bytes)) {
+    (void)hipGetLastError();
+    t->mem = nullptr;
+    return set_error(TEMPI_ERR_NOMEM, "no scratch for trial copies (%zu B)",
+                     bytes);
+  }
+  char *base = static_cast<char *>(t->mem);
+  char *image =
+      base + (((uintptr_t)first - (uintptr_t)base) & (uintptr_t)255);
+  if (dir == TEMPI_DIR_UNPACK) {
+    t->strided = image - g.start - lo;
+    t->packed = packed;
+  } else {
+    t->strided = strided;
+    t->packed = image;
+  }
+  return TEMPI_SUCCESS;
+}
+
 // The default policy for copies timed cold (see kColdBytes): the model's
 // three best variants and the closed-form rule's pick, each timed once on
-// the caller's buffers (a pack or unpack rewrites the same bytes, so the
-// trial copies change nothing), the fastest kept for the shape.  About a
-// millisecond once per shape, against 10-20 % that a wrong pick costs each
-// call of a copy this size.
+// the copy, the fastest kept for the shape.  About a millisecond once per
+// shape, against 10-20 % that a wrong pick costs each call of a copy this
+// size.  Smaller copies keep the model's pick: a warm trial times the copy
+// repeated on its own, and which variant wins there is not which wins where
+// the copy runs -- a variant with nontemporal stores that times best alone
+// sends the packed bytes out of the caches just before the transfer or
+// unpack that reads them.
+
 static int verify_candidates(const Geom &g, const char *strided, char *packed,
                              int dir, int rule, hipStream_t stream,
                              Choice *out) {
@@ -204,10 +285,15 @@ static int verify_candidates(const Geom &g, const char *strided, char *packed,
   if (std::find(cand, cand + n, rule) == cand + n) {
     cand[n++] = rule;
   }
+  TrialBuffers t;
+  int err = trial_buffers(g, strided, packed, dir, stream, &t);
+  if (err) {
+    return err;
+  }
   float best_us = 1e30f;
   for (int i = 0; i < n; ++i) {
     float us = 0;
-    int err = time_config(g, strided, packed, dir, cand[i], 0, 1, stream, &us);
+    err = time_config(g, t.strided, t.packed, dir, cand[i], 0, 1, stream, &us);
     if (err) {
       return err;
     }
@@ -262,7 +348,10 @@ Choice choose(const Geom &g, const char *strided, char *packed, int dir,
     }
   }
   Choice best = c;
-  if (autotune(g, strided, packed, dir, stream, &best, nullptr)) {
+  TrialBuffers t;
+  if (trial_buffers(g, strided, packed, dir, stream, &t) ||
+      autotune(g, t.strided, t.packed, dir, stream, &best, nullptr) ||
+      hipSuccess != hipStreamSynchronize(stream)) {
     TEMPI_LOG(LOG_WARN, "autotune failed (%s); using the closed-form rule",
               tempi_last_error());
     return c;

@@ -217,82 +217,51 @@ static int batch_launch_on(tempi_batch *b, hipStream_t s) {
   return TEMPI_SUCCESS;
 }
 
-TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
-                                    int method, tempi_batch **out) {
-  if (!out || (n > 0 && !reqs) || n < 0) {
-    return set_error(TEMPI_ERR_ARG, "bad argument");
+namespace {
+
+// One request as the batch resolves it.
+struct Req {
+  Geom g;
+  const char *ds = nullptr; // strided buffer origin (device address)
+  char *dp = nullptr;       // first packed byte (device address)
+  int dir = 0;
+  int variant = 0;
+};
+
+// The device tables and launch shape of a batch of resolved requests;
+// replaces what `b` held (waiting for the library stream first).
+int build_batch(tempi_batch *b, const std::vector<Req> &rs) {
+  hipStream_t s = library_stream();
+  if (b->dev) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(b->dev);
+    b->dev = nullptr;
   }
-  *out = nullptr;
-  int err = ensure_runtime();
-  if (err) {
-    return err;
-  }
-  tempi_batch *b = new tempi_batch();
-  b->requests = n;
+  b->small = false;
+  b->xcd_split = 0;
+  b->ix = SmallIndex{};
   std::vector<BatchItem> items;
   std::vector<uint32_t> item_tiles; // tiles of each item
   uint64_t tiles = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    const tempi_batch_request &r = reqs[i];
-    b->variants.push_back(-1);
-    Geom g;
-    if ((err = make_geom(r.sb, r.count, &g))) {
-      delete b;
-      return err;
-    }
-    if (g.rows == 0 || g.L == 0) {
-      continue;
-    }
-    if (g.L > (1ll << 30)) {
-      delete b;
-      return set_error(TEMPI_ERR_UNSUPPORTED,
-                       "request %lld: rows over 1 GiB are not batched",
-                       (long long)i);
-    }
-    void *ds = nullptr, *dp = nullptr;
-    if (!device_address(r.strided, &ds) || !device_address(r.packed, &dp)) {
-      delete b;
-      return set_error(TEMPI_ERR_ARG,
-                       "request %lld: buffers must be device-accessible",
-                       (long long)i);
-    }
-    const int dir = r.direction == TEMPI_DIR_UNPACK ? 1 : 0;
-    int variant = r.variant;
-    if (variant < 0 || variant >= kNumVariants) {
-      // The autotuner and the learned selector rank variants by the time of
-      // one launch of the copy alone.  Below about a megabyte that time is
-      // launch latency, which says nothing about what the copy's tiles cost
-      // inside a shared launch; such requests take the closed-form rule,
-      // which sizes tiles for throughput.
-      const int m = g.L * g.rows < kBatchSelectMinBytes ? TEMPI_SELECT_HEURISTIC
-                                                         : method;
-      variant = choose(g, static_cast<const char *>(ds), static_cast<char *>(dp),
-                       dir, m, library_stream())
-                    .variant;
-    }
-    b->variants.back() = variant;
+  for (const Req &r : rs) {
     BatchItem it{};
     bool aligned = false;
-    if ((err = fill_kparams(g, static_cast<const char *>(ds),
-                            static_cast<char *>(dp), dir, variant, &it.p,
-                            &aligned))) {
-      delete b;
+    int err = fill_kparams(r.g, r.ds, r.dp, r.dir, r.variant, &it.p, &aligned);
+    if (err) {
       return err;
     }
-    it.kind = 4 * variant + 2 * (aligned ? 1 : 0) + dir;
-    const VariantDesc &v = kVariants[variant];
+    it.kind = 4 * r.variant + 2 * (aligned ? 1 : 0) + r.dir;
+    const VariantDesc &v = kVariants[r.variant];
     it.units_per_tile = (uint32_t)(kBlock / v.P) * units_in_flight(v.U);
     const uint64_t t = ((uint64_t)it.p.units + it.units_per_tile - 1) /
                        it.units_per_tile;
     tiles += t;
     if (tiles >= (1ull << 31)) {
-      delete b;
       return set_error(TEMPI_ERR_UNSUPPORTED, "batch exceeds 2^31 tiles");
     }
     it.ntiles = (uint32_t)t;
     item_tiles.push_back((uint32_t)t);
     items.push_back(it);
-    b->bytes += g.L * g.rows;
   }
   b->n = (int)items.size();
   b->ntiles = (uint32_t)tiles;
@@ -335,24 +304,23 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
       const uint64_t n_r = item_tiles[r];
       const uint64_t lo = n_r * x / groups, hi = n_r * (x + 1) / groups;
       for (uint64_t l = lo; l < hi; ++l) {
-        tile_list.push_back(
-            Tile{(uint32_t)r, (uint32_t)l});
+        tile_list.push_back(Tile{(uint32_t)r, (uint32_t)l});
       }
     }
   }
   const size_t items_bytes = items.size() * sizeof(BatchItem);
   const size_t tl_off = (items_bytes + 255) / 256 * 256;
-  const size_t xb_off = tl_off + (tile_list.size() * sizeof(Tile) + 255) / 256 * 256;
+  const size_t xb_off =
+      tl_off + (tile_list.size() * sizeof(Tile) + 255) / 256 * 256;
   const size_t total = xb_off + 256;
   if (hipSuccess != hipMalloc(&b->dev, total)) {
     (void)hipGetLastError();
-    delete b;
+    b->dev = nullptr;
     return set_error(TEMPI_ERR_NOMEM, "batch table allocation failed");
   }
   b->d_items = reinterpret_cast<BatchItem *>(b->dev);
   b->d_tiles = reinterpret_cast<Tile *>(b->dev + tl_off);
   b->d_xcd_begin = reinterpret_cast<uint32_t *>(b->dev + xb_off);
-  hipStream_t s = library_stream();
   std::vector<char> host(total, 0);
   if (!items.empty()) {
     memcpy(host.data(), items.data(), items_bytes);
@@ -368,8 +336,79 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
       hipSuccess != hipStreamSynchronize(s)) {
     (void)hipGetLastError();
     (void)hipFree(b->dev);
-    delete b;
+    b->dev = nullptr;
     return set_error(TEMPI_ERR_HIP, "batch table upload failed");
+  }
+  return TEMPI_SUCCESS;
+}
+
+} // namespace
+
+TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
+                                    int method, tempi_batch **out) {
+  if (!out || (n > 0 && !reqs) || n < 0) {
+    return set_error(TEMPI_ERR_ARG, "bad argument");
+  }
+  *out = nullptr;
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  hipStream_t s = library_stream();
+  tempi_batch *b = new tempi_batch();
+  b->requests = n;
+  b->variants.assign((size_t)n, -1);
+  std::vector<Req> rs;
+  std::vector<int64_t> where; // caller's index of each resolved request
+  for (int64_t i = 0; i < n; ++i) {
+    const tempi_batch_request &r = reqs[i];
+    Req q;
+    if ((err = make_geom(r.sb, r.count, &q.g))) {
+      delete b;
+      return err;
+    }
+    if (q.g.rows == 0 || q.g.L == 0) {
+      continue;
+    }
+    if (q.g.L > (1ll << 30)) {
+      delete b;
+      return set_error(TEMPI_ERR_UNSUPPORTED,
+                       "request %lld: rows over 1 GiB are not batched",
+                       (long long)i);
+    }
+    void *ds = nullptr, *dp = nullptr;
+    if (!device_address(r.strided, &ds) || !device_address(r.packed, &dp)) {
+      delete b;
+      return set_error(TEMPI_ERR_ARG,
+                       "request %lld: buffers must be device-accessible",
+                       (long long)i);
+    }
+    q.ds = static_cast<const char *>(ds);
+    q.dp = static_cast<char *>(dp);
+    q.dir = r.direction == TEMPI_DIR_UNPACK ? 1 : 0;
+    q.variant = r.variant;
+    if (q.variant < 0 || q.variant >= kNumVariants) {
+      // The autotuner and the learned selector rank variants by the time of
+      // one launch of the copy alone.  Below about a megabyte that time is
+      // launch latency, which says nothing about what the copy's tiles cost
+      // inside a shared launch; such requests take the closed-form rule,
+      // which sizes tiles for throughput.  Larger requests take the
+      // per-copy policy.
+      const int m = q.g.L * q.g.rows < kBatchSelectMinBytes
+                        ? TEMPI_SELECT_HEURISTIC
+                        : method;
+      q.variant = choose(q.g, q.ds, q.dp, q.dir, m, s).variant;
+    }
+    b->bytes += q.g.L * q.g.rows;
+    rs.push_back(q);
+    where.push_back(i);
+  }
+  for (size_t k = 0; k < rs.size(); ++k) {
+    b->variants[(size_t)where[k]] = rs[k].variant;
+  }
+  if ((err = build_batch(b, rs))) {
+    delete b;
+    return err;
   }
   *out = b;
   return TEMPI_SUCCESS;

@@ -207,6 +207,7 @@ TEMPI_EXPORT int tempi_counters_get(tempi_counters *out) {
   out->p2p_oneshot = c.p2p_oneshot;
   out->p2p_staged = c.p2p_staged;
   out->p2p_library = c.p2p_library;
+  out->trial_launches = c.trial_launches;
   return TEMPI_SUCCESS;
 }
 
@@ -227,6 +228,7 @@ TEMPI_EXPORT int tempi_counters_reset(void) {
   c.p2p_oneshot = 0;
   c.p2p_staged = 0;
   c.p2p_library = 0;
+  c.trial_launches = 0;
   return TEMPI_SUCCESS;
 }
 

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #define TEMPI_EXPORT extern "C" __attribute__((visibility("default")))
@@ -83,8 +84,18 @@ struct Counters {
   std::atomic<int64_t> p2p_oneshot{0};
   std::atomic<int64_t> p2p_staged{0};
   std::atomic<int64_t> p2p_library{0}; // forwarded to the MPI library
+  std::atomic<int64_t> trial_launches{0}; // copies timed by the library
 };
 Counters &counters();
+
+// While a TrialScope lives on a thread, the copies it launches are the
+// library's timing runs (verification, autotuning, system tables): they
+// count as trial launches, not as the caller's pack or unpack kernels.
+struct TrialScope {
+  TrialScope();
+  ~TrialScope();
+};
+bool in_trial();
 
 // ---------------------------------------------------------------------------
 // the underlying MPI library (reference: include/symbols.hpp:180-215,
@@ -231,6 +242,36 @@ int model_ranked(const Geom &g, const char *strided, const char *packed,
                  int dir, int k, int *out);
 // copies of at least this many packed bytes are timed cold (autotune.hip)
 bool times_cold(const Geom &g);
+bool times_cold_bytes(int64_t bytes);
+// sweep `bytes` of a library-owned buffer so the caches hold nothing of the
+// next timed launch (autotune.hip)
+int flush_caches_on(int64_t bytes, hipStream_t s);
+constexpr int64_t kColdFlushBytesAll = 512ll << 20;
+
+// Bytes from the first to the last byte a copy writes: the packed stream
+// for a pack, the span of the strided rows for an unpack.
+int64_t written_span(const Geom &g, int dir);
+
+// A scratch image of the side a copy writes, for timing runs that must not
+// touch the caller's buffers: `strided` / `packed` are the copy's buffers
+// with the written one replaced by the image, which has the written span's
+// length and its address modulo 256.  The image is freed once `stream` has
+// finished (the destructor synchronizes it); it is not kept in the staging
+// pool, which would hold on to memory a one-time trial no longer needs.
+struct TrialBuffers {
+  void *mem = nullptr;
+  hipStream_t stream = nullptr;
+  const char *strided = nullptr;
+  char *packed = nullptr;
+  TrialBuffers() = default;
+  TrialBuffers(const TrialBuffers &) = delete;
+  TrialBuffers &operator=(const TrialBuffers &) = delete;
+  TrialBuffers(TrialBuffers &&o) noexcept { *this = std::move(o); }
+  TrialBuffers &operator=(TrialBuffers &&o) noexcept;
+  ~TrialBuffers();
+};
+int trial_buffers(const Geom &g, const char *strided, char *packed, int dir,
+                  hipStream_t stream, TrialBuffers *t);
 
 // system performance tables (perf.hip): load <cache dir>/perf.json if present
 void perf_init();

@@ -254,13 +254,23 @@ static int launch_one(const Geom &g, const char *strided, char *packed,
   TEMPI_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void *>(fn),
                                   dim3((unsigned)blocks), dim3(kBlock), args,
                                   0, stream));
-  if (dir == TEMPI_DIR_PACK) {
+  if (in_trial()) {
+    ++counters().trial_launches;
+  } else if (dir == TEMPI_DIR_PACK) {
     ++counters().pack_kernel_launches;
   } else {
     ++counters().unpack_kernel_launches;
   }
   return TEMPI_SUCCESS;
 }
+
+namespace {
+thread_local int t_trial_depth = 0;
+} // namespace
+
+TrialScope::TrialScope() { ++t_trial_depth; }
+TrialScope::~TrialScope() { --t_trial_depth; }
+bool in_trial() { return t_trial_depth > 0; }
 
 int launch_copy(const Geom &g, const char *strided, char *packed, int dir,
                 int variant, hipStream_t stream, int max_blocks) {

@@ -649,6 +649,7 @@ int measure_copies(int which, double max_s, std::vector<IidTime> *out) {
 // min(bytes, 2^j) bytes at a 512-B stride, as MPI_Pack runs it (the variant
 // the library's selector picks, then a stream synchronization)
 int measure_table(int table, double max_s, Table *out) {
+  TrialScope trial; // the table's copies are measurement, not the caller's
   const int64_t max_bytes = 1ll << (2 * (TEMPI_PERF_TABLE_DIM - 1) + 6);
   const int64_t stride = 512;
   const bool host = table == TEMPI_TABLE_PACK_HOST ||

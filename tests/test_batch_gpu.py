@@ -231,3 +231,42 @@ def test_small_batches(k):
         else:
             orc.sb_unpack(desc, phost[pk:pk + sz], exp_s, so, 1)
     assert (got_s == exp_s).all(), k
+
+
+def test_batch_creation_leaves_buffers_alone():
+    """Creating a batch settles the variants of its large requests by timing
+    trial copies; they write scratch, not the requests' buffers.  An unpack
+    request's strided buffer and a pack request's packed buffer keep their
+    contents until the batch runs, and then hold exactly the copy."""
+    assert ta.lib().tempi_mlp_load(None) == 0
+    c0 = ta.counters()
+    # 64 MiB of 1-KiB rows at a 1088-B stride (a shape no other test uses)
+    rows = 65536
+    sb = ta.StridedBlock.make(16, 16 + (rows - 1) * 1088 + 1024, [1024, rows], [1, 1088])
+    desc = orc.sb_desc(*sb.as_tuple())
+    n = sb.extent + 64
+    host = orc.splitmix_bytes(n, 61)
+    phost = orc.splitmix_bytes(sb.size(), 62)
+    src = ta.DeviceBuffer(n)
+    src.upload(host)
+    out = ta.DeviceBuffer(sb.size())
+    out.fill(0x5A)
+    pk = ta.DeviceBuffer(sb.size())
+    pk.upload(phost)
+    dst = ta.DeviceBuffer(n)
+    dst.fill(0xAB)
+    b = ta.Batch([(sb, src.addr(), out.addr(), 1, ta.DIR_PACK),
+                  (sb, dst.addr(), pk.addr(), 1, ta.DIR_UNPACK)])
+    ta.synchronize()
+    c1 = ta.counters()
+    assert c1["verified_choices"] >= c0["verified_choices"] + 2
+    assert c1["trial_launches"] > c0["trial_launches"]
+    assert c1["pack_kernel_launches"] == c0["pack_kernel_launches"]
+    assert (out.download() == 0x5A).all(), "pack trial wrote the packed buffer"
+    assert (dst.download() == 0xAB).all(), "unpack trial wrote the strided buffer"
+    b.launch()
+    ta.synchronize()
+    assert (out.download() == orc.sb_pack(desc, host, 0, 1)).all()
+    ref = np.full(n, 0xAB, np.uint8)
+    orc.sb_unpack(desc, phost, ref, 0, 1)
+    assert (dst.download() == ref).all()

@@ -244,9 +244,10 @@ def test_alignment_shape_sweep_sample():
 
 def test_default_policy_settles_large_copies():
     """A copy of 64 MiB or more under the default policy: the first call
-    times a few candidates on the caller's buffers and keeps the fastest for
-    the shape, later calls reuse it; every call's bytes stay exact (packs
-    against the oracle, unpacks keep 0xAB outside the type map)."""
+    times a few candidates (into a scratch image of the destination) and
+    keeps the fastest for the shape, later calls reuse it; every call's
+    bytes stay exact (packs against the oracle, unpacks keep 0xAB outside
+    the type map)."""
     assert ta.lib().tempi_mlp_load(None) == 0
     bl, stride, rows = 256, 576, (64 << 20) // 256
     sb = ta.StridedBlock.make(0, (rows - 1) * stride + bl, [bl, rows], [1, stride])
