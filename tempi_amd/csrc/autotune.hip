@@ -229,6 +229,9 @@ int64_t written_span(const Geom &g, int dir) {
   return hi - lo;
 }
 
+// largest scratch image a trial may allocate
+constexpr size_t kMaxTrialScratch = 4ull << 30;
+
 int trial_buffers(const Geom &g, const char *strided, char *packed, int dir,
                   hipStream_t stream, TrialBuffers *t) {
   // bytes the copy writes: the packed stream, or the strided rows' span
@@ -245,6 +248,12 @@ int trial_buffers(const Geom &g, const char *strided, char *packed, int dir,
   }
   const size_t bytes = (size_t)(hi - lo) + 256;
   t->stream = stream;
+  if (bytes > kMaxTrialScratch) {
+    // a sparse unpack can span far more than it writes; its trials are not
+    // worth that much memory, and the copy keeps the model's pick
+    return set_error(TEMPI_ERR_NOMEM,
+                     "trial copies would need %zu B of scratch", bytes);
+  }
   if (hipSuccess != hipMalloc(&t->mem, bytes)) {
     (void)hipGetLastError();
     t->mem = nullptr;
