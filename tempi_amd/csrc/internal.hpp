@@ -275,6 +275,8 @@ int trial_buffers(const Geom &g, const char *strided, char *packed, int dir,
 
 // system performance tables (perf.hip): load <cache dir>/perf.json if present
 void perf_init();
+// changes whenever the measured tables do (perf.hip)
+uint64_t perf_generation();
 // modelled seconds of a method (perf.hip); +inf when a table is missing
 double perf_model_seconds(int method, bool colocated, int64_t bytes,
                           int64_t block_length);

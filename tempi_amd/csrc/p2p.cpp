@@ -95,11 +95,17 @@ bool colocated(MPI_Comm comm, int peer) {
 // method choice
 // ---------------------------------------------------------------------------
 std::map<std::tuple<bool, int64_t, int64_t>, int> g_method_cache;
+uint64_t g_method_cache_generation = 0; // of the tables the cache was built on
 
 int choose_method(const Canon &cn, int64_t bytes, MPI_Comm comm, int peer) {
   const int m = forced_datatype_method();
   if (m != TEMPI_METHOD_UNKNOWN) {
     return m;
+  }
+  if (g_method_cache_generation != perf_generation()) {
+    // tables loaded, measured or replaced since: price again
+    g_method_cache.clear();
+    g_method_cache_generation = perf_generation();
   }
   const int64_t bl = std::min<int64_t>(std::max<int64_t>(1, cn.sb.counts[0]), 512);
   const bool near = colocated(comm, peer);

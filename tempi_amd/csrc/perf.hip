@@ -35,6 +35,7 @@
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <atomic>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -81,6 +82,9 @@ constexpr int kMaxCols = 32;
 std::mutex g_perf_mu;
 Performance g_perf;
 std::map<std::tuple<int, int64_t, int64_t>, int> g_method_cache;
+// bumped whenever the tables change, so that choices cached elsewhere
+// (the point-to-point method cache) are dropped
+std::atomic<uint64_t> g_perf_generation{0};
 
 int log2_floor(int64_t x) { return x <= 0 ? 0 : 63 - __builtin_clzll(x); }
 int log2_ceil(int64_t x) {
@@ -740,6 +744,8 @@ int check_table_id(int t) {
 
 } // namespace
 
+uint64_t perf_generation() { return g_perf_generation.load(); }
+
 double perf_model_seconds(int method, bool colocated, int64_t bytes,
                           int64_t block_length) {
   std::lock_guard<std::mutex> lk(g_perf_mu);
@@ -785,6 +791,7 @@ TEMPI_EXPORT int tempi_perf_clear(void) {
   std::lock_guard<std::mutex> lk(g_perf_mu);
   g_perf = Performance();
   g_method_cache.clear();
+  ++g_perf_generation;
   return TEMPI_SUCCESS;
 }
 
@@ -820,6 +827,7 @@ TEMPI_EXPORT int tempi_perf_set_curve(int curve, const double *times, int n) {
     c[i].time = times[i];
   }
   g_method_cache.clear();
+  ++g_perf_generation;
   return TEMPI_SUCCESS;
 }
 
@@ -866,6 +874,7 @@ TEMPI_EXPORT int tempi_perf_set_table(int table, const double *times, int rows,
     }
   }
   g_method_cache.clear();
+  ++g_perf_generation;
   return TEMPI_SUCCESS;
 }
 
@@ -896,6 +905,7 @@ TEMPI_EXPORT int tempi_perf_load(const char *path) {
   std::lock_guard<std::mutex> lk(g_perf_mu);
   g_perf = std::move(perf);
   g_method_cache.clear();
+  ++g_perf_generation;
   TEMPI_LOG(LOG_INFO, "loaded system performance from %s", p.c_str());
   return TEMPI_SUCCESS;
 }
@@ -972,6 +982,7 @@ TEMPI_EXPORT int tempi_perf_measure(int what, double seconds_per_point) {
   }
   std::lock_guard<std::mutex> lk(g_perf_mu);
   g_method_cache.clear();
+  ++g_perf_generation;
   return TEMPI_SUCCESS;
 }
 

@@ -360,3 +360,37 @@ def test_two_ranks_device_buffers(tmp_path):
     assert p.returncode == 0, out
     for r in (0, 1):
         assert f"rank {r} done" in out, out
+
+
+def test_method_choice_follows_replaced_tables(mpi):
+    """With no method forced, a send of device data takes the method the
+    measured tables price cheapest; replacing the tables re-prices the next
+    send of the same size (cached choices belong to the tables they were
+    made on)."""
+    from tempi_amd import system as tsys
+    ta.lib().tempi_set_datatype_method(0)  # by model
+    h, src, dst, expected = setup(mpi, FACE, 1, 23)
+
+    def tables(pack_host_s, pack_device_s):
+        tsys.clear()
+        for c in (tsys.D2H, tsys.H2D, tsys.INTRA_CPU_CPU):
+            tsys.set_curve(c, np.full(tsys.CURVE_LEN, 1e-6))
+        tsys.set_table(tsys.PACK_HOST, np.full((9, 9), pack_host_s))
+        tsys.set_table(tsys.UNPACK_HOST, np.full((9, 9), pack_host_s))
+        tsys.set_table(tsys.PACK_DEVICE, np.full((9, 9), pack_device_s))
+        tsys.set_table(tsys.UNPACK_DEVICE, np.full((9, 9), pack_device_s))
+
+    try:
+        for cheap, key in (("host", "p2p_oneshot"), ("device", "p2p_staged")):
+            tables(1e-6 if cheap == "host" else 1.0, 1e-6 if cheap == "device" else 1.0)
+            dst.fill(0xAB)
+            c0 = ta.counters()[key]
+            err, rr = mpi.irecv(dst.addr(), 1, h, 0, 61)
+            err, sr = mpi.isend(src.addr(), 1, h, 0, 61)
+            err, _, _ = mpi.waitall([rr, sr], [h, h])
+            assert err == 0
+            assert ta.counters()[key] == c0 + 2, (cheap, ta.counters())
+            assert (dst.download() == expected).all(), cheap
+    finally:
+        tsys.clear()
+        mpi.free(h)
