@@ -648,7 +648,8 @@ def bench_p2p(ta, m, faces, grid, grid2, host, reps=10):
     import numpy as np
 
     res = {}
-    names = {2: "oneshot", 3: "staged"}
+    # 0: the library's own choice (the method model when it is installed)
+    names = {2: "oneshot", 3: "staged", 0: "library_choice"}
     for method, name in names.items():
         ta.lib().tempi_set_datatype_method(method)
         ts = []
@@ -666,6 +667,12 @@ def bench_p2p(ta, m, faces, grid, grid2, host, reps=10):
         us = float(np.median(ts)) * 1e6
         res[name] = {"us_per_exchange": round(us, 1),
                      "packed_gbs": round(sum(f["size"] for f in faces) / us / 1e3, 2)}
+        if method == 0:
+            res[name]["method_per_face"] = [
+                {1: "device", 2: "oneshot", 3: "staged"}.get(
+                    ta.lib().tempi_p2p_method(1, f["size"], f["sb"].as_tuple()[2][0],
+                                              f["sb"].as_tuple()[3][1]), "unknown")
+                for f in faces]
     ta.lib().tempi_set_datatype_method(-1)
     dst = np.zeros_like(host)
     ts = []

@@ -336,6 +336,7 @@ typedef struct tempi_counters {
   int64_t p2p_staged;
   int64_t p2p_library;          /* point-to-point calls forwarded as they are */
   int64_t trial_launches;       /* copies the library timed (not the caller's) */
+  int64_t method_model_choices; /* staging methods chosen by the method MLP */
 } tempi_counters;
 int tempi_counters_get(tempi_counters *out);
 int tempi_counters_reset(void);
@@ -419,6 +420,31 @@ int tempi_choose_method(int colocated, int64_t bytes, int64_t block_length);
    TEMPI_DATATYPE_{ONESHOT,DEVICE,AUTO} variables (src/internal/env.cpp:
    73-81), for tools and tests. */
 int tempi_set_datatype_method(int method);
+/* The staging method the library takes for a message of `bytes` packed
+   bytes in rows of `block` bytes `stride` apart (TEMPI_METHOD_*): forced by
+   tempi_set_datatype_method or TEMPI_DATATYPE_*; else, for a peer on the
+   same node, the method model's cheapest; else the measured tables'
+   cheapest (reference: src/internal/sender.cpp:128-273, which always uses
+   the tables). */
+int tempi_p2p_method(int colocated, int64_t bytes, int64_t block,
+                     int64_t stride);
+
+/* Learned staging-method model: no counterpart in the reference, whose
+   method choice sums measured table entries (sender.cpp:128-273).  The
+   selector's network (tempi_mlp_*) with its own weights: output k predicts
+   the time of method k + 1 (DEVICE, ONESHOT, STAGED; log2, relative),
+   trained on end-to-end timings of each method by tempi_amd/method_train.py.
+   tempi_method_features: the network's 16 inputs for a message (log2 bytes,
+     row bytes, rows, stride, stride / row bytes; same node; log2 alignment of
+     the row length; the rest 0).
+   tempi_method_model_load: weights file (NULL: method_mlp.txt next to the
+     library; $TEMPI_METHOD_MODEL overrides the default); the library loads
+     it on first use.  tempi_method_model_unload: back to the tables. */
+int tempi_method_features(int64_t bytes, int64_t block, int64_t stride,
+                          int colocated, float *features);
+int tempi_method_model_load(const char *path);
+int tempi_method_model_unload(void);
+int tempi_method_model_loaded(void);
 
 #ifdef __cplusplus
 }

@@ -112,6 +112,7 @@ class Counters(ctypes.Structure):
             "p2p_staged",
             "p2p_library",
             "trial_launches",
+            "method_model_choices",
         )
     ]
 
@@ -214,6 +215,12 @@ def _prototypes(L):
         "tempi_perf_measure": (_I, [_I, ctypes.c_double]),
         "tempi_perf_model": (_I, [_I, _I, _I64, _I64, ctypes.POINTER(ctypes.c_double)]),
         "tempi_choose_method": (_I, [_I, _I64, _I64]),
+        "tempi_p2p_method": (_I, [_I, _I64, _I64, _I64]),
+        "tempi_set_datatype_method": (_I, [_I]),
+        "tempi_method_features": (_I, [_I64, _I64, _I64, _I, ctypes.POINTER(ctypes.c_float)]),
+        "tempi_method_model_load": (_I, [ctypes.c_char_p]),
+        "tempi_method_model_unload": (_I, []),
+        "tempi_method_model_loaded": (_I, []),
     }
     for name, (res, args) in protos.items():
         fn = getattr(L, name)

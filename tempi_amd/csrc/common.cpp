@@ -102,6 +102,11 @@ void read_environment() {
     e.contiguous_auto = true;
   }
   e.gpu_aware_mpi = nullptr != std::getenv("TEMPI_MPI_GPU_AWARE");
+  if (const char *m = std::getenv("TEMPI_METHOD_MODEL")) {
+    e.method_model_path = m;
+  } else {
+    e.method_model_path.clear();
+  }
   if (const char *m = std::getenv("TEMPI_MODEL")) {
     e.model_path = m;
   } else {
@@ -208,6 +213,7 @@ TEMPI_EXPORT int tempi_counters_get(tempi_counters *out) {
   out->p2p_staged = c.p2p_staged;
   out->p2p_library = c.p2p_library;
   out->trial_launches = c.trial_launches;
+  out->method_model_choices = c.method_model_choices;
   return TEMPI_SUCCESS;
 }
 
@@ -229,6 +235,7 @@ TEMPI_EXPORT int tempi_counters_reset(void) {
   c.p2p_staged = 0;
   c.p2p_library = 0;
   c.trial_launches = 0;
+  c.method_model_choices = 0;
   return TEMPI_SUCCESS;
 }
 

@@ -60,6 +60,7 @@ struct Env {
   int select = TEMPI_SELECT_DEFAULT; // TEMPI_SELECT=heuristic|model|autotune
   std::string cache_dir;       // TEMPI_CACHE_DIR (autotune cache, model)
   std::string model_path;      // TEMPI_MODEL (MLP weights file)
+  std::string method_model_path; // TEMPI_METHOD_MODEL (method MLP weights)
 };
 Env &env();
 void read_environment();
@@ -85,6 +86,7 @@ struct Counters {
   std::atomic<int64_t> p2p_staged{0};
   std::atomic<int64_t> p2p_library{0}; // forwarded to the MPI library
   std::atomic<int64_t> trial_launches{0}; // copies timed by the library
+  std::atomic<int64_t> method_model_choices{0}; // staging methods the MLP chose
 };
 Counters &counters();
 
@@ -236,6 +238,16 @@ bool model_available();
 int model_variant(const Geom &g, const char *strided, const char *packed,
                   int dir, const std::string &key); // -1: no model
 void model_precommit(const tempi_strided_block &sb);
+// inputs of the selector networks (variants and staging methods)
+constexpr int kModelInputs = 16;
+// the staging-method model (mlp.hip): features of a message of `bytes`
+// packed bytes in rows of `block` bytes `stride` apart, and the candidate the
+// model prices cheapest (-1 without a model)
+void method_features(int64_t bytes, int64_t block, int64_t stride,
+                     bool colocated, float *f);
+int method_model_pick(const float *f, const int *candidates, int n);
+// changes whenever the method model is loaded or unloaded (mlp.hip)
+uint64_t method_model_generation();
 // the model's k best variants for this copy, best first (not cached);
 // returns how many were written (0 without a model)
 int model_ranked(const Geom &g, const char *strided, const char *packed,

@@ -37,6 +37,7 @@ namespace {
 constexpr int kOut = 0 TEMPI_VARIANTS(TEMPI_COUNT_V);
 #undef TEMPI_COUNT_V
 constexpr int kIn = 16, kH1 = 64, kH2 = 64;
+static_assert(kIn == kModelInputs, "one input width for both networks");
 constexpr int kOffW1 = 0;
 constexpr int kOffB1 = kOffW1 + kIn * kH1;
 constexpr int kOffW2 = kOffB1 + kH1;
@@ -395,16 +396,21 @@ static int model_infer(Model &m, const float *x, int n, int *best,
   return TEMPI_SUCCESS;
 }
 
-static std::string default_model_path() {
+// `name` in the directory the library was loaded from
+static std::string library_file(const char *name) {
   Dl_info info{};
-  if (dladdr(reinterpret_cast<void *>(&default_model_path), &info) &&
+  if (dladdr(reinterpret_cast<void *>(&library_file), &info) &&
       info.dli_fname) {
     std::string p(info.dli_fname);
     const size_t slash = p.rfind('/');
     return (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) +
-           "/selector_mlp.txt";
+           "/" + name;
   }
-  return "selector_mlp.txt";
+  return name;
+}
+
+static std::string default_model_path() {
+  return library_file("selector_mlp.txt");
 }
 
 static int load_model_file(Model &m, const std::string &path) {
@@ -541,6 +547,92 @@ void model_precommit(const tempi_strided_block &sb) {
       m->cache[shape_key_of(g, base, pk, d)] = best[d];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// the staging-method model: the same network, another set of weights
+// ---------------------------------------------------------------------------
+//
+// The reference prices each way of moving a non-contiguous message (DEVICE,
+// ONE_SHOT, STAGED) as a sum of measured table entries (pack + transfer +
+// unpack, src/internal/sender.cpp:128-273).  Here a second instance of the
+// selector's network is trained on end-to-end timings of each method on
+// device-resident derived types (tempi_amd/method_train.py): output k
+// predicts method k + 1's time (log2, relative to the fastest measured for
+// the message), and the cheapest of the methods that can run is taken.
+// Features (the rest zero): log2 packed bytes, row bytes, rows, stride and
+// stride / row bytes; peer on the same node; log2 alignment of the row
+// length.
+
+namespace {
+Model &method_model_slot() {
+  static Model m;
+  return m;
+}
+std::atomic<uint64_t> g_method_model_generation{0};
+} // namespace
+
+uint64_t method_model_generation() { return g_method_model_generation.load(); }
+
+void method_features(int64_t bytes, int64_t block, int64_t stride,
+                     bool colocated, float *f) {
+  for (int i = 0; i < kIn; ++i) {
+    f[i] = 0.f;
+  }
+  const double b = (double)std::max<int64_t>(bytes, 1);
+  const double bl = (double)std::max<int64_t>(block, 1);
+  const double st = (double)std::max<int64_t>(stride, block);
+  f[0] = log2p(b);
+  f[1] = log2p(bl);
+  f[2] = log2p(b / bl);
+  f[3] = log2p(st);
+  f[4] = log2p(st / bl);
+  f[5] = colocated ? 1.f : 0.f;
+  f[6] = (float)align_log2((uint64_t)std::max<int64_t>(block, 1));
+}
+
+// $TEMPI_METHOD_MODEL, else method_mlp.txt next to the library, loaded on
+// first use; nullptr if there is none
+static Model *loaded_method_model() {
+  ensure_env();
+  Model &m = method_model_slot();
+  std::lock_guard<std::mutex> lk(m.mu);
+  if (!m.tried) {
+    m.tried = true;
+    const std::string path = env().method_model_path.empty()
+                                 ? library_file("method_mlp.txt")
+                                 : env().method_model_path;
+    if (load_model_file(m, path)) {
+      TEMPI_LOG(LOG_INFO, "no method model (%s): measured tables",
+                tempi_last_error());
+    }
+  }
+  return m.loaded ? &m : nullptr;
+}
+
+int method_model_pick(const float *f, const int *candidates, int n) {
+  Model *m = loaded_method_model();
+  if (!m || n <= 0 || ensure_runtime()) {
+    return -1;
+  }
+  float pred[kOut];
+  int best = -1;
+  {
+    std::lock_guard<std::mutex> lk(m->mu);
+    if (model_infer(*m, f, 1, &best, pred)) {
+      return -1;
+    }
+  }
+  int pick = -1;
+  float lowest = INFINITY;
+  for (int i = 0; i < n; ++i) {
+    const int k = candidates[i] - 1; // output k: method k + 1
+    if (k >= 0 && k < kOut && pred[k] < lowest) {
+      lowest = pred[k];
+      pick = candidates[i];
+    }
+  }
+  return pick;
 }
 
 } // namespace tempi
@@ -758,3 +850,40 @@ TEMPI_EXPORT int tempi_mlp_load(const char *path) {
 }
 
 TEMPI_EXPORT int tempi_mlp_loaded(void) { return model_available() ? 1 : 0; }
+
+TEMPI_EXPORT int tempi_method_features(int64_t bytes, int64_t block,
+                                       int64_t stride, int colocated,
+                                       float *features) {
+  if (!features || bytes < 0 || block < 0) {
+    return set_error(TEMPI_ERR_ARG, "bad argument");
+  }
+  method_features(bytes, block, stride, colocated != 0, features);
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_method_model_load(const char *path) {
+  ensure_env();
+  Model &m = method_model_slot();
+  std::lock_guard<std::mutex> lk(m.mu);
+  m.tried = true;
+  const int err =
+      load_model_file(m, path ? path : library_file("method_mlp.txt"));
+  if (err) {
+    m.loaded = false;
+  }
+  ++g_method_model_generation;
+  return err;
+}
+
+TEMPI_EXPORT int tempi_method_model_unload(void) {
+  Model &m = method_model_slot();
+  std::lock_guard<std::mutex> lk(m.mu);
+  m.tried = true;
+  m.loaded = false;
+  ++g_method_model_generation;
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_method_model_loaded(void) {
+  return loaded_method_model() ? 1 : 0;
+}

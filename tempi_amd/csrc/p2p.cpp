@@ -94,22 +94,31 @@ bool colocated(MPI_Comm comm, int peer) {
 // ---------------------------------------------------------------------------
 // method choice
 // ---------------------------------------------------------------------------
-std::map<std::tuple<bool, int64_t, int64_t>, int> g_method_cache;
-uint64_t g_method_cache_generation = 0; // of the tables the cache was built on
+std::map<std::tuple<bool, int64_t, int64_t, int64_t>, int> g_method_cache;
+// the tables and method model the cache was built on
+uint64_t g_method_cache_tables = 0, g_method_cache_model = 0;
 
-int choose_method(const Canon &cn, int64_t bytes, MPI_Comm comm, int peer) {
+// The staging method for a message of `bytes` packed bytes in rows of
+// `block` bytes `stride` apart, to a peer on this node or not: forced by the
+// environment or tempi_set_datatype_method; else, for a peer on this node,
+// the method model's cheapest (mlp.hip: trained on end-to-end timings of
+// each method on this kind of node); else the measured tables' cheapest
+// (reference: sender.cpp:128-273); without either, DEVICE when the MPI
+// library reads device memory, otherwise ONESHOT.  Cached per message shape.
+int choose_method_for(bool near, int64_t bytes, int64_t block,
+                      int64_t stride) {
   const int m = forced_datatype_method();
   if (m != TEMPI_METHOD_UNKNOWN) {
     return m;
   }
-  if (g_method_cache_generation != perf_generation()) {
-    // tables loaded, measured or replaced since: price again
+  if (g_method_cache_tables != perf_generation() ||
+      g_method_cache_model != method_model_generation()) {
+    // tables or method model replaced since: price again
     g_method_cache.clear();
-    g_method_cache_generation = perf_generation();
+    g_method_cache_tables = perf_generation();
+    g_method_cache_model = method_model_generation();
   }
-  const int64_t bl = std::min<int64_t>(std::max<int64_t>(1, cn.sb.counts[0]), 512);
-  const bool near = colocated(comm, peer);
-  const auto key = std::make_tuple(near, bytes, bl);
+  const auto key = std::make_tuple(near, bytes, block, stride);
   auto it = g_method_cache.find(key);
   if (it != g_method_cache.end()) {
     return it->second;
@@ -118,17 +127,35 @@ int choose_method(const Canon &cn, int64_t bytes, MPI_Comm comm, int peer) {
   if (env().gpu_aware_mpi) {
     cands.insert(cands.begin(), TEMPI_METHOD_DEVICE);
   }
-  int best = env().gpu_aware_mpi ? TEMPI_METHOD_DEVICE : TEMPI_METHOD_ONESHOT;
-  double best_s = INFINITY;
-  for (int c : cands) {
-    const double s = perf_model_seconds(c, near, bytes, bl);
-    if (s < best_s) {
-      best_s = s;
-      best = c;
+  int best = -1;
+  if (near) {
+    float f[kModelInputs];
+    method_features(bytes, block, stride, near, f);
+    best = method_model_pick(f, cands.data(), (int)cands.size());
+    if (best > 0) {
+      ++counters().method_model_choices;
+    }
+  }
+  if (best <= 0) {
+    best = env().gpu_aware_mpi ? TEMPI_METHOD_DEVICE : TEMPI_METHOD_ONESHOT;
+    const int64_t bl = std::min<int64_t>(std::max<int64_t>(1, block), 512);
+    double best_s = INFINITY;
+    for (int c : cands) {
+      const double t = perf_model_seconds(c, near, bytes, bl);
+      if (t < best_s) {
+        best_s = t;
+        best = c;
+      }
     }
   }
   g_method_cache[key] = best;
   return best;
+}
+
+int choose_method(const Canon &cn, int64_t bytes, MPI_Comm comm, int peer) {
+  const int64_t block = std::max<int64_t>(1, cn.sb.counts[0]);
+  const int64_t stride = cn.sb.ndims >= 2 ? cn.sb.strides[1] : block;
+  return choose_method_for(colocated(comm, peer), bytes, block, stride);
 }
 
 void count_method(int m) {
@@ -611,6 +638,15 @@ void p2p_finalize() {
 } // namespace tempi
 
 using namespace tempi;
+
+TEMPI_EXPORT int tempi_p2p_method(int colocated, int64_t bytes, int64_t block,
+                                 int64_t stride) {
+  if (bytes < 0 || block < 0) {
+    return TEMPI_METHOD_UNKNOWN;
+  }
+  ensure_env();
+  return choose_method_for(colocated != 0, bytes, block, stride);
+}
 
 TEMPI_EXPORT int tempi_set_datatype_method(int method) {
   if (method != -1 && method != TEMPI_METHOD_UNKNOWN &&

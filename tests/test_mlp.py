@@ -120,3 +120,38 @@ def test_model_file_round_trip_and_rejects(tmp_path):
     assert ta.select_variant(sb, 1, 1 << 20, 1 << 20, method=ta.SELECT_MODEL) == \
         ta.select_variant(sb, 1, 1 << 20, 1 << 20, method=ta.SELECT_HEURISTIC)
     L.tempi_mlp_load(None)  # back to the default model, if one is installed
+
+
+def test_method_features_follow_their_definition():
+    """The staging-method network's inputs: log2 bytes, row bytes, rows,
+    stride and stride / row bytes; same node; log2 alignment of the row
+    length (capped at 16 B); zeros after."""
+    L = ta.lib()
+    f = (ctypes.c_float * 16)()
+    assert L.tempi_method_features(4096, 64, 192, 1, f) == 0
+    exp = [12, 6, 6, math.log2(192), math.log2(3), 1, 4] + [0] * 9
+    np.testing.assert_allclose(list(f), exp, rtol=1e-6)
+    # 12-B rows of a float grid's x face, another node
+    assert L.tempi_method_features(3 << 20, 12, 2072, 0, f) == 0
+    exp = [math.log2(3 << 20), math.log2(12), math.log2((3 << 20) / 12), math.log2(2072),
+           math.log2(2072 / 12), 0, 2] + [0] * 9
+    np.testing.assert_allclose(list(f), exp, rtol=1e-6)
+    assert L.tempi_method_features(-1, 12, 2072, 0, f) == ta.TEMPI_ERR_ARG
+
+
+def test_method_model_file_is_checked_like_the_selector(tmp_path):
+    """The method model is the selector's network with its own weights: a
+    file of the wrong shape is refused and the library keeps the tables."""
+    L = ta.lib()
+    params = mlp_ref.init_params(OUT, 9)
+    norm = np.concatenate([np.zeros(16, np.float32), np.ones(16, np.float32)])
+    path = str(tmp_path / "method.txt")
+    assert L.tempi_mlp_save(path.encode(), params.ctypes.data, norm.ctypes.data) == 0
+    bad = tmp_path / "bad.txt"
+    bad.write_text(open(path).read().replace(f"out {OUT}", f"out {OUT - 1}"))
+    assert L.tempi_method_model_load(str(bad).encode()) == ta.TEMPI_ERR_IO
+    assert L.tempi_method_model_loaded() == 0
+    assert L.tempi_method_model_load(path.encode()) == 0
+    assert L.tempi_method_model_loaded() == 1
+    assert L.tempi_method_model_unload() == 0 and L.tempi_method_model_loaded() == 0
+    L.tempi_method_model_load(None)  # back to the installed model, if any

@@ -394,3 +394,55 @@ def test_method_choice_follows_replaced_tables(mpi):
     finally:
         tsys.clear()
         mpi.free(h)
+
+
+def test_method_model_drives_the_choice(mpi, tmp_path):
+    """With no method forced, a send to a peer on this node takes the method
+    the method model prices cheapest: two crafted weight files (all weights
+    zero, output biases favouring ONE_SHOT in one and STAGED in the other)
+    each decide the method of the next exchange, whose bytes arrive intact;
+    loading a model re-prices shapes chosen before it."""
+    L = ta.lib()
+    n, out = L.tempi_mlp_num_params(), L.tempi_mlp_num_outputs()
+    norm = np.concatenate([np.zeros(16, np.float32), np.ones(16, np.float32)])
+    ta.lib().tempi_set_datatype_method(0)
+    h, src, dst, expected = setup(mpi, FACE, 1, 29)
+    try:
+        for fast, key in ((ONESHOT, "p2p_oneshot"), (STAGED, "p2p_staged")):
+            params = np.zeros(n, np.float32)
+            params[n - out:] = 1.0
+            params[n - out + fast - 1] = 0.0  # output k: method k + 1
+            path = str(tmp_path / f"method{fast}.txt")
+            assert L.tempi_mlp_save(path.encode(), params.ctypes.data, norm.ctypes.data) == 0
+            assert L.tempi_method_model_load(path.encode()) == 0
+            c0 = ta.counters()
+            assert L.tempi_p2p_method(1, 4096, 64, 192) == fast
+            dst.fill(0xAB)
+            err, rr = mpi.irecv(dst.addr(), 1, h, 0, 71)
+            err, sr = mpi.isend(src.addr(), 1, h, 0, 71)
+            err, _, _ = mpi.waitall([rr, sr], [h, h])
+            assert err == 0
+            c1 = ta.counters()
+            assert c1[key] == c0[key] + 2, (fast, c1)
+            assert c1["method_model_choices"] > c0["method_model_choices"]
+            assert (dst.download() == expected).all(), fast
+        # a peer on another node is priced by the tables, not the model
+        c0 = ta.counters()["method_model_choices"]
+        L.tempi_p2p_method(0, 4096, 64, 192)
+        assert ta.counters()["method_model_choices"] == c0
+    finally:
+        L.tempi_method_model_load(None)  # the installed model, or none
+        mpi.free(h)
+
+
+def test_installed_method_model_is_consulted():
+    """The shipped method model (tempi_amd/method_mlp.txt) loads, and the
+    library asks it for a same-node message whose method is not forced."""
+    L = ta.lib()
+    assert L.tempi_method_model_load(None) == 0, "tempi_amd/method_mlp.txt is missing"
+    ta.lib().tempi_set_datatype_method(0)
+    c0 = ta.counters()["method_model_choices"]
+    # a shape no other test prices, so the choice is not cached
+    m = L.tempi_p2p_method(1, 12345 * 1024, 1024, 3000)
+    assert m in (ONESHOT, STAGED)
+    assert ta.counters()["method_model_choices"] == c0 + 1
