@@ -363,12 +363,13 @@ def test_two_ranks_device_buffers(tmp_path):
 
 
 def test_method_choice_follows_replaced_tables(mpi):
-    """With no method forced, a send of device data takes the method the
-    measured tables price cheapest; replacing the tables re-prices the next
-    send of the same size (cached choices belong to the tables they were
-    made on)."""
+    """With no method forced and no method model, a send of device data takes
+    the method the measured tables price cheapest; replacing the tables
+    re-prices the next send of the same size (cached choices belong to the
+    tables they were made on)."""
     from tempi_amd import system as tsys
     ta.lib().tempi_set_datatype_method(0)  # by model
+    ta.lib().tempi_method_model_unload()  # the tables decide
     h, src, dst, expected = setup(mpi, FACE, 1, 23)
 
     def tables(pack_host_s, pack_device_s):
@@ -393,6 +394,7 @@ def test_method_choice_follows_replaced_tables(mpi):
             assert (dst.download() == expected).all(), cheap
     finally:
         tsys.clear()
+        ta.lib().tempi_method_model_load(None)  # the installed model, if any
         mpi.free(h)
 
 
