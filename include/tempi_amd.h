@@ -398,10 +398,18 @@ int tempi_perf_save(const char *path);
 /* Fill the empty entries this GPU can measure (reference: measure_performance,
    src/internal/measure_system.cu:381-609): bit 0 kernel launch, bit 1 the
    d2h/h2d curves, bit 2 the device pack/unpack tables, bit 3 the mapped-host
-   (ONE_SHOT) tables.  Ping-pong curves need two ranks and are only
-   imported.  Each point takes at least 7 samples and at most
+   (ONE_SHOT) tables.  Each point takes at least 7 samples and at most
    `seconds_per_point` (<= 0: 0.05 s) or 500 samples; its time is their
-   trimean (src/internal/statistics.cpp:30-39). */
+   trimean (src/internal/statistics.cpp:30-39).
+   Bit 4 (TEMPI_MEASURE_PINGPONG) is collective over MPI_COMM_WORLD and needs
+   at least two ranks: ranks 0 and 1 bounce host messages of 1 B - 8 MiB
+   through the MPI library (reference: the CPU-CPU ping-pong of
+   bin/measure_system.cpp), 7 samples of a repetition count rank 0 picks for
+   about 200 us, one-way time = trimean / 2; the curve, broadcast to every
+   rank, fills intraNodeCpuCpuPingpong when the two ranks share a node, else
+   interNodeCpuCpuPingpong.  It needs no GPU.  The GPU-GPU curves need a
+   GPU-aware MPI library and stay imported. */
+#define TEMPI_MEASURE_PINGPONG 16
 int tempi_perf_measure(int what, double seconds_per_point);
 /* Modelled time of sending `bytes` of non-contiguous data in blocks of
    `block_length` by `method`; TEMPI_ERR_UNSUPPORTED when an input is missing.

@@ -309,6 +309,8 @@ void pool_trim();
 // point-to-point state (p2p.cpp)
 void p2p_init();     // at MPI_Init: which ranks share a node
 void p2p_finalize(); // at MPI_Finalize: completes or reports pending work
+// do two ranks of MPI_COMM_WORLD share a node (by the map MPI_Init built)?
+bool world_ranks_share_node(int a, int b);
 // the staging method forced by tempi_set_datatype_method or TEMPI_DATATYPE_*
 // (DEVICE becomes STAGED without a GPU-aware library); UNKNOWN: by model
 int forced_datatype_method();

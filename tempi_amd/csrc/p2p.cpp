@@ -602,6 +602,14 @@ int forced_datatype_method() {
   return m;
 }
 
+bool world_ranks_share_node(int a, int b) {
+  if (a < 0 || b < 0 || a >= (int)g_node_of_world_rank.size() ||
+      b >= (int)g_node_of_world_rank.size()) {
+    return g_node_of_world_rank.size() <= 1;
+  }
+  return g_node_of_world_rank[a] == g_node_of_world_rank[b];
+}
+
 void p2p_init() {
   int size = 1;
   if (MPI_SUCCESS != MPI_Comm_size(MPI_COMM_WORLD, &size) || size <= 1) {

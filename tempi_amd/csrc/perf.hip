@@ -649,6 +649,64 @@ int measure_copies(int which, double max_s, std::vector<IidTime> *out) {
   return TEMPI_SUCCESS;
 }
 
+// one-way CPU-CPU message time between world ranks 0 and 1, collective over
+// MPI_COMM_WORLD (reference: bin/measure_system.cpp, the CPU-CPU ping-pong);
+// rank 0 picks each size's repetition count so that every rank runs the same
+// loop, and broadcasts the curve
+int measure_pingpong(int *curve, std::vector<IidTime> *out) {
+  int rank = 0, size = 1;
+  MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+  MPI_Comm_size(MPI_COMM_WORLD, &size);
+  if (size < 2) {
+    return set_error(TEMPI_ERR_UNSUPPORTED, "ping-pong needs two ranks");
+  }
+  *curve = world_ranks_share_node(0, 1) ? TEMPI_CURVE_INTRA_CPU_CPU
+                                        : TEMPI_CURVE_INTER_CPU_CPU;
+  const int n_max = 1 << (TEMPI_PERF_CURVE_LEN - 1);
+  std::vector<char> buf((size_t)n_max, 1);
+  const RealMpi &mpi = real_mpi();
+  constexpr int kTag = 27717, kSamples = 7;
+  auto bounce = [&](int n, int reps) {
+    for (int r = 0; r < reps; ++r) {
+      if (rank == 0) {
+        mpi.Send(buf.data(), n, MPI_BYTE, 1, kTag, MPI_COMM_WORLD);
+        mpi.Recv(buf.data(), n, MPI_BYTE, 1, kTag, MPI_COMM_WORLD,
+                 MPI_STATUS_IGNORE);
+      } else if (rank == 1) {
+        mpi.Recv(buf.data(), n, MPI_BYTE, 0, kTag, MPI_COMM_WORLD,
+                 MPI_STATUS_IGNORE);
+        mpi.Send(buf.data(), n, MPI_BYTE, 0, kTag, MPI_COMM_WORLD);
+      }
+    }
+  };
+  std::vector<double> one_way(TEMPI_PERF_CURVE_LEN, 0.0);
+  for (int i = 0; i < TEMPI_PERF_CURVE_LEN; ++i) {
+    const int n = 1 << i;
+    int reps = 1;
+    const auto w = Clock::now();
+    bounce(n, 1); // warm-up, and the round trip that sizes the samples
+    if (rank == 0) {
+      reps = std::max(1, std::min(10000, (int)(200e-6 / std::max(
+                                              seconds_since(w), 1e-9))));
+    }
+    MPI_Bcast(&reps, 1, MPI_INT, 0, MPI_COMM_WORLD);
+    std::vector<double> xs;
+    for (int k = 0; k < kSamples; ++k) {
+      const auto t0 = Clock::now();
+      bounce(n, reps);
+      xs.push_back(seconds_since(t0) / reps / 2);
+    }
+    one_way[i] = trimean(xs);
+  }
+  MPI_Bcast(one_way.data(), TEMPI_PERF_CURVE_LEN, MPI_DOUBLE, 0,
+            MPI_COMM_WORLD);
+  out->clear();
+  for (double t : one_way) {
+    out->push_back(IidTime{t, false});
+  }
+  return TEMPI_SUCCESS;
+}
+
 // a synchronous pack or unpack of 2^(2i+6) bytes in blocks of
 // min(bytes, 2^j) bytes at a 512-B stride, as MPI_Pack runs it (the variant
 // the library's selector picks, then a stream synchronization)
@@ -933,11 +991,11 @@ TEMPI_EXPORT int tempi_perf_save(const char *path) {
 }
 
 TEMPI_EXPORT int tempi_perf_measure(int what, double seconds_per_point) {
-  if (what & ~0xF) {
+  if (what & ~0x1F) {
     return set_error(TEMPI_ERR_ARG, "measure mask 0x%x", what);
   }
-  int err = ensure_runtime();
-  if (err) {
+  int err = TEMPI_SUCCESS;
+  if ((what & 0xF) && (err = ensure_runtime())) {
     return err;
   }
   const double max_s = seconds_per_point > 0 ? seconds_per_point : 0.05;
@@ -946,6 +1004,18 @@ TEMPI_EXPORT int tempi_perf_measure(int what, double seconds_per_point) {
   {
     std::lock_guard<std::mutex> lk(g_perf_mu);
     cur = g_perf;
+  }
+  if (what & TEMPI_MEASURE_PINGPONG) {
+    // collective: every rank measures, whatever its tables hold
+    int c = 0;
+    std::vector<IidTime> v;
+    if ((err = measure_pingpong(&c, &v))) {
+      return err;
+    }
+    std::lock_guard<std::mutex> lk(g_perf_mu);
+    if (g_perf.curves[c].empty()) {
+      g_perf.curves[c] = std::move(v);
+    }
   }
   if ((what & 1) && cur.kernel_launch == 0) {
     double t = 0;

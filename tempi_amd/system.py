@@ -31,6 +31,8 @@ METHOD_UNKNOWN, METHOD_DEVICE, METHOD_ONESHOT, METHOD_STAGED = range(4)
 
 MEASURE_LAUNCH, MEASURE_COPIES, MEASURE_DEVICE_TABLES, MEASURE_HOST_TABLES = 1, 2, 4, 8
 MEASURE_ALL = 15
+# collective over MPI_COMM_WORLD, at least two ranks; needs no GPU
+MEASURE_PINGPONG = 16
 
 
 def _dptr(a):

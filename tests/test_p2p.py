@@ -120,3 +120,51 @@ def test_two_ranks_host_buffers(tmp_path):
                        capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "rank 1 ok" in r.stdout
+
+
+def test_two_ranks_measure_the_pingpong_curve(tmp_path):
+    """Two processes under mpiexec (CPU only) measure the CPU-CPU ping-pong
+    curve together (TEMPI_MEASURE_PINGPONG): both end up with the same 24
+    points (rank 0's measurement, broadcast), finite and positive, filed as
+    the intra-node curve since the ranks share this node, and 8 MiB takes
+    longer than 1 B."""
+    import os
+    import shutil
+    import subprocess
+    import sys
+
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if not os.path.exists(mpiexec):
+        import pytest
+        pytest.skip("no mpiexec")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = tmp_path / "pingpong.py"
+    script.write_text(
+        "import sys, ctypes\n"
+        f"sys.path.insert(0, {root!r})\n"
+        "import numpy as np\n"
+        "import tempi_amd as ta\n"
+        "from tempi_amd import mpi as tmpi, system\n"
+        "m = tmpi.get()\n"
+        "rank = ctypes.c_int()\n"
+        "m.tempi.MPI_Comm_rank.argtypes = [m.H, ctypes.POINTER(ctypes.c_int)]\n"
+        "m.tempi.MPI_Comm_rank(m.COMM_WORLD, ctypes.byref(rank))\n"
+        "system.clear()\n"
+        "system.measure_performance(system.MEASURE_PINGPONG, 0.01)\n"
+        "c = system.get_curve(system.INTRA_CPU_CPU)\n"
+        "assert system.get_curve(system.INTER_CPU_CPU).size == 0\n"
+        "assert c.shape == (24,) and np.isfinite(c).all() and (c > 0).all(), c\n"
+        "assert c[23] > c[0], c\n"
+        "print(f'rank {rank.value} curve ' + ' '.join(f'{x:.9e}' for x in c), flush=True)\n"
+        "m.finalize()\n")
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="", ROCR_VISIBLE_DEVICES="")
+    r = subprocess.run([mpiexec, "-n", "2", "-hosts", "127.0.0.1", sys.executable, str(script)],
+                       capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    curves = {}
+    for line in r.stdout.splitlines():
+        if " curve " in line:
+            who, vals = line.split(" curve ")
+            curves[who] = vals
+    assert set(curves) == {"rank 0", "rank 1"}, r.stdout
+    assert curves["rank 0"] == curves["rank 1"]
