@@ -654,7 +654,12 @@ int measure_copies(int which, double max_s, std::vector<IidTime> *out) {
 // rank 0 picks each size's repetition count so that every rank runs the same
 // loop, and broadcasts the curve
 int measure_pingpong(int *curve, std::vector<IidTime> *out) {
-  int rank = 0, size = 1;
+  int rank = 0, size = 1, up = 0, down = 0;
+  MPI_Initialized(&up);
+  MPI_Finalized(&down);
+  if (!up || down) {
+    return set_error(TEMPI_ERR_UNSUPPORTED, "ping-pong needs MPI initialized");
+  }
   MPI_Comm_rank(MPI_COMM_WORLD, &rank);
   MPI_Comm_size(MPI_COMM_WORLD, &size);
   if (size < 2) {

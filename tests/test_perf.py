@@ -204,7 +204,9 @@ def test_argument_errors():
     assert L.tempi_perf_set_curve(6, None, 0) == ta.TEMPI_ERR_ARG
     assert L.tempi_perf_set_table(-1, None, 0, 0) == ta.TEMPI_ERR_ARG
     assert L.tempi_perf_set_table(0, None, 17, 9) == ta.TEMPI_ERR_ARG
-    assert L.tempi_perf_measure(0x10, 0.0) == ta.TEMPI_ERR_ARG
+    assert L.tempi_perf_measure(0x20, 0.0) == ta.TEMPI_ERR_ARG
+    # the ping-pong needs a second rank
+    assert L.tempi_perf_measure(0x10, 0.0) == ta.TEMPI_ERR_UNSUPPORTED
     assert tsys.model(7, True, 1, 1) is None
 
 
