@@ -3,6 +3,10 @@
     python -m tempi_amd.method_train --out tempi_amd/method_mlp.txt \
         --report tempi_amd/method_mlp.report.json [--train 3000] [--heldout 1000]
 
+Under `mpiexec -n 2` the two ranks also measure the CPU-CPU ping-pong curve
+first, and the report then compares the model with the reference's own
+choice priced on fully measured tables.
+
 The reference picks how a non-contiguous message travels -- packed on the
 device and handed over as a device pointer (DEVICE), packed straight into
 pinned host memory (ONE_SHOT), or packed on the device and copied to the host
@@ -23,8 +27,8 @@ what the choice is about instead: the end-to-end time of each method.
    to the faster method, outputs of methods not timed masked out.
 4. On held-out messages, judged on an independent second timing, report the
    regret of the model's choice, of always ONE_SHOT (the choice without
-   tables) and of always STAGED, and write the weights file the library
-   loads.
+   tables), of always STAGED, and -- with the ping-pong curve -- of the
+   table sum; write the weights file the library loads.
 """
 from __future__ import annotations
 
@@ -41,6 +45,7 @@ import numpy as np
 
 import tempi_amd as ta
 from tempi_amd import mpi as tmpi
+from tempi_amd import system
 
 SPAN = 256 << 20  # largest strided span of a message
 DEVICE, ONESHOT, STAGED = 1, 2, 3
@@ -148,6 +153,23 @@ def main(argv=None):
     L = ta.lib()
     assert ta.device_count() > 0, "no GPU"
     m = tmpi.get()
+    # Under mpiexec with two or more ranks, the ranks first measure the
+    # CPU-CPU ping-pong curve together, so that rank 0 can price the
+    # reference's table sum on measured tables too; the other ranks then
+    # leave.
+    rank, size = ctypes.c_int(), ctypes.c_int()
+    m.tempi.MPI_Comm_rank.argtypes = [m.H, ctypes.POINTER(ctypes.c_int)]
+    m.tempi.MPI_Comm_size.argtypes = [m.H, ctypes.POINTER(ctypes.c_int)]
+    m.tempi.MPI_Comm_rank(m.COMM_WORLD, ctypes.byref(rank))
+    m.tempi.MPI_Comm_size(m.COMM_WORLD, ctypes.byref(size))
+    have_pingpong = False
+    if size.value >= 2:
+        system.clear()
+        system.measure_performance(system.MEASURE_PINGPONG, 0.01)
+        have_pingpong = True
+        if rank.value != 0:
+            m.finalize()
+            return 0
     # the library hands device pointers to the MPI library only when told it
     # reads device memory
     gpu_aware = os.environ.get("TEMPI_MPI_GPU_AWARE") is not None
@@ -165,6 +187,18 @@ def main(argv=None):
     timing_s = time.time() - t0
     src.free()
     dst.free()
+
+    # the reference's choice: the cheapest sum of measured table entries
+    # (only with the ping-pong curve, which needs a second rank)
+    table_pick = None
+    if have_pingpong:
+        system.measure_performance(system.MEASURE_ALL, 0.01)
+        L.tempi_method_model_unload()
+        ta.lib().tempi_set_datatype_method(-1)
+        col = {meth: j for j, meth in enumerate(methods)}
+        table_pick = np.array([col.get(L.tempi_p2p_method(1, packed, block, stride), 0)
+                               for _, packed, block, stride in hcases])
+        L.tempi_method_model_load(None)
 
     nf, nout = L.tempi_mlp_num_features(), L.tempi_mlp_num_outputs()
     mean = Xtr.mean(0)
@@ -223,6 +257,8 @@ def main(argv=None):
         r = {"model": regret(T, pm_ho)}
         for k, v in fixed.items():
             r[k] = regret(T, v)
+        if table_pick is not None:
+            r["measured_tables"] = regret(T, table_pick)
         if name == "heldout_retimed":
             # the faster method of the first timing, judged on the second
             r["measured_best_first_timing"] = regret(T, Tho.argmin(1))
@@ -230,6 +266,8 @@ def main(argv=None):
             r["seconds_total"] = {"model": float(T[idx, pm_ho].sum()),
                                   "best": float(T.min(1).sum()),
                                   **{k: float(T[idx, v].sum()) for k, v in fixed.items()}}
+            if table_pick is not None:
+                r["seconds_total"]["measured_tables"] = float(T[idx, table_pick].sum())
         report[name] = r
     # where the methods trade places: per size band of the held-out set,
     # median time of each method and how often each is the faster

@@ -109,7 +109,9 @@ def export_performance(path: str | None = None):
 
 
 def measure_performance(what: int = MEASURE_ALL, seconds_per_point: float = 0.05):
-    """Fill the empty entries this GPU can measure (needs a GPU)."""
+    """Fill the empty entries this GPU can measure (needs a GPU); with
+    MEASURE_PINGPONG, collectively with every rank of MPI_COMM_WORLD, the
+    CPU-CPU ping-pong curve (two ranks or more, no GPU needed)."""
     check(lib().tempi_perf_measure(what, seconds_per_point), "perf_measure")
 
 
