@@ -61,6 +61,42 @@ def test_isend_irecv_self(mpi, method):
 
 
 @pytest.mark.parametrize("method", [ONESHOT, STAGED])
+def test_wire_format_is_the_libraries_packed_form(mpi, method):
+    """A message travels as MPI_PACKED bytes in type-map order, the MPI
+    library's own packed form: a device-buffer side (packed or unpacked here)
+    talks to a host-buffer side whose derived type the MPI library handles
+    itself, in both directions (`sender.cpp:109`,
+    `async_operation.cpp:127,261`)."""
+    ta.lib().tempi_set_datatype_method(method)
+    h, src, dst, expected = setup(mpi, FACE, 2, 41)
+    host_src = src.download()
+    c0 = ta.counters()
+    # device sender -> host receiver: the library unpacks the packed bytes
+    host_dst = np.full(expected.size, 0xAB, np.uint8)
+    err, rr = mpi.irecv(host_dst.ctypes.data, 2, h, 0, 51)
+    assert err == 0
+    err, sr = mpi.isend(src.addr(), 2, h, 0, 51)
+    assert err == 0
+    err, sts, reqs = mpi.waitall([rr, sr], [h, h])
+    assert err == 0 and reqs == [mpi.REQUEST_NULL] * 2
+    assert sts[0]["count"] == 2
+    assert (host_dst == expected).all()
+    # host sender -> device receiver: the library packs, the kernel unpacks
+    err, rr = mpi.irecv(dst.addr(), 2, h, 0, 52)
+    assert err == 0
+    err, sr = mpi.isend(host_src.ctypes.data, 2, h, 0, 52)
+    assert err == 0
+    err, sts, reqs = mpi.waitall([rr, sr], [h, h])
+    assert err == 0 and reqs == [mpi.REQUEST_NULL] * 2
+    assert sts[0]["count"] == 2
+    assert (dst.download() == expected).all()
+    c1 = ta.counters()
+    assert c1[COUNTER[method]] == c0[COUNTER[method]] + 2
+    assert c1["p2p_library"] == c0["p2p_library"] + 2
+    mpi.free(h)
+
+
+@pytest.mark.parametrize("method", [ONESHOT, STAGED])
 def test_blocking_send_and_recv_self(mpi, method):
     ta.lib().tempi_set_datatype_method(method)
     spec = ("vector", 4096, 48, 200, tc.BYTE)
