@@ -381,12 +381,16 @@ __global__ void k_empty(int *p) {
 using namespace tempi;
 
 static int resolve_buffers(const tempi_strided_block *sb, int64_t incount,
-                           const void *strided, const void *packed, Geom *g,
-                           void **ds, void **dp) {
+                           const void *strided, const void *packed,
+                           int direction, Geom *g, void **ds, void **dp) {
   if (!sb) {
     return set_error(TEMPI_ERR_ARG, "null strided block");
   }
-  int err = make_geom(*sb, incount, g);
+  int err = check_direction(direction);
+  if (err) {
+    return err;
+  }
+  err = make_geom(*sb, incount, g);
   if (err) {
     return err;
   }
@@ -408,7 +412,8 @@ TEMPI_EXPORT int tempi_time_variant(const tempi_strided_block *sb,
                                     int grid_cap, int reps, float *us) {
   Geom g;
   void *ds = nullptr, *dp = nullptr;
-  int err = resolve_buffers(sb, incount, strided, packed, &g, &ds, &dp);
+  int err = resolve_buffers(sb, incount, strided, packed, direction, &g, &ds,
+                            &dp);
   if (err) {
     return err;
   }
@@ -426,7 +431,8 @@ TEMPI_EXPORT int tempi_autotune(const tempi_strided_block *sb, int64_t incount,
                                 int *best_grid_cap, float *per_variant_us) {
   Geom g;
   void *ds = nullptr, *dp = nullptr;
-  int err = resolve_buffers(sb, incount, strided, packed, &g, &ds, &dp);
+  int err = resolve_buffers(sb, incount, strided, packed, direction, &g, &ds,
+                            &dp);
   if (err) {
     return err;
   }

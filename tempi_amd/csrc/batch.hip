@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 namespace tempi {
@@ -350,12 +351,21 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
     return set_error(TEMPI_ERR_ARG, "bad argument");
   }
   *out = nullptr;
-  int err = ensure_runtime();
-  if (err) {
+  // every request is checked before the device is touched
+  int err = TEMPI_SUCCESS;
+  std::vector<Geom> geoms((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    if ((err = check_direction(reqs[i].direction)) ||
+        (err = make_geom(reqs[i].sb, reqs[i].count, &geoms[(size_t)i]))) {
+      return err;
+    }
+  }
+  if ((err = ensure_runtime())) {
     return err;
   }
   hipStream_t s = library_stream();
-  tempi_batch *b = new tempi_batch();
+  std::unique_ptr<tempi_batch> owner(new tempi_batch());
+  tempi_batch *b = owner.get();
   b->requests = n;
   b->variants.assign((size_t)n, -1);
   std::vector<Req> rs;
@@ -363,29 +373,24 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   for (int64_t i = 0; i < n; ++i) {
     const tempi_batch_request &r = reqs[i];
     Req q;
-    if ((err = make_geom(r.sb, r.count, &q.g))) {
-      delete b;
-      return err;
-    }
+    q.g = geoms[(size_t)i];
     if (q.g.rows == 0 || q.g.L == 0) {
       continue;
     }
     if (q.g.L > (1ll << 30)) {
-      delete b;
       return set_error(TEMPI_ERR_UNSUPPORTED,
                        "request %lld: rows over 1 GiB are not batched",
                        (long long)i);
     }
     void *ds = nullptr, *dp = nullptr;
     if (!device_address(r.strided, &ds) || !device_address(r.packed, &dp)) {
-      delete b;
       return set_error(TEMPI_ERR_ARG,
                        "request %lld: buffers must be device-accessible",
                        (long long)i);
     }
     q.ds = static_cast<const char *>(ds);
     q.dp = static_cast<char *>(dp);
-    q.dir = r.direction == TEMPI_DIR_UNPACK ? 1 : 0;
+    q.dir = r.direction;
     q.variant = r.variant;
     if (q.variant < 0 || q.variant >= kNumVariants) {
       // The autotuner and the learned selector rank variants by the time of
@@ -407,10 +412,9 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
     b->variants[(size_t)where[k]] = rs[k].variant;
   }
   if ((err = build_batch(b, rs))) {
-    delete b;
     return err;
   }
-  *out = b;
+  *out = owner.release();
   return TEMPI_SUCCESS;
 }
 

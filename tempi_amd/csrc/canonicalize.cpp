@@ -376,7 +376,9 @@ void normalize_dims(std::vector<Dim> &dims) {
     // dims[i] continues exactly where dims[i-1] ends: one longer dimension.
     // For the byte run (stride 1) this is stride == bytes.
     for (size_t i = 1; i < dims.size(); ++i) {
-      if (dims[i].stride == dims[i - 1].count * dims[i - 1].stride) {
+      int64_t end = 0;
+      if (!__builtin_mul_overflow(dims[i - 1].count, dims[i - 1].stride, &end) &&
+          dims[i].stride == end) {
         dims[i - 1].count *= dims[i].count;
         dims.erase(dims.begin() + i);
         changed = true;

@@ -34,6 +34,14 @@ void log_message(int level, const char *fmt, ...)
 int set_error(int code, const char *fmt, ...)
     __attribute__((format(printf, 2, 3)));
 
+// TEMPI_DIR_PACK or TEMPI_DIR_UNPACK; any other value is an argument error
+// (it would otherwise select a copy direction silently)
+inline int check_direction(int dir) {
+  return dir == TEMPI_DIR_PACK || dir == TEMPI_DIR_UNPACK
+             ? TEMPI_SUCCESS
+             : set_error(TEMPI_ERR_ARG, "direction %d", dir);
+}
+
 #define TEMPI_HIP_CHECK(expr)                                                  \
   do {                                                                         \
     hipError_t e_ = (expr);                                                    \

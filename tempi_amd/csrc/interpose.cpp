@@ -296,12 +296,17 @@ TEMPI_EXPORT int tempi_variant_info_get(int variant, tempi_variant_info *info) {
   return TEMPI_SUCCESS;
 }
 
-static int64_t sb_size(const tempi_strided_block &sb) {
-  int64_t s = 1;
-  for (int d = 0; d < sb.ndims; ++d) {
-    s *= sb.counts[d];
+// packed bytes of `count` objects of `sb`, with the copy's own validation
+// of the descriptor (dimension count, signs, 64-bit products)
+static int packed_bytes(const tempi_strided_block &sb, int64_t count,
+                        int64_t *bytes) {
+  Geom g;
+  int err = make_geom(sb, count, &g);
+  if (err) {
+    return err;
   }
-  return s;
+  *bytes = g.L * g.rows;
+  return TEMPI_SUCCESS;
 }
 
 TEMPI_EXPORT int tempi_select_variant(const tempi_strided_block *sb,
@@ -311,13 +316,17 @@ TEMPI_EXPORT int tempi_select_variant(const tempi_strided_block *sb,
   if (!sb || !variant) {
     return set_error(TEMPI_ERR_ARG, "null argument");
   }
+  int err = check_direction(direction);
+  if (err) {
+    return err;
+  }
   Geom g;
-  int err = make_geom(*sb, incount, &g);
+  err = make_geom(*sb, incount, &g);
   if (err) {
     return err;
   }
   if (method == TEMPI_SELECT_AUTOTUNE) {
-    int err = ensure_runtime();
+    err = ensure_runtime();
     if (err) {
       return err;
     }
@@ -341,15 +350,19 @@ TEMPI_EXPORT int tempi_pack_sb(const tempi_strided_block *sb,
                                const void *inbuf, int64_t incount,
                                void *outbuf, int64_t outsize,
                                int64_t *position, int variant, void *stream) {
-  if (!sb || !position || incount < 0 || *position < 0) {
+  if (!sb || !position || incount < 0 || *position < 0 || outsize < 0) {
     return set_error(TEMPI_ERR_ARG, "bad argument");
   }
-  const int64_t packed = incount * sb_size(*sb);
-  if (*position + packed > outsize) {
+  int64_t packed = 0;
+  int err = packed_bytes(*sb, incount, &packed);
+  if (err) {
+    return err;
+  }
+  if (packed > outsize - *position) {
     return set_error(TEMPI_ERR_TRUNCATE, "need %lld bytes, have %lld",
                      (long long)(*position + packed), (long long)outsize);
   }
-  int err = ensure_runtime();
+  err = ensure_runtime();
   if (err) {
     return err;
   }
@@ -373,15 +386,19 @@ TEMPI_EXPORT int tempi_unpack_sb(const tempi_strided_block *sb,
                                  const void *inbuf, int64_t insize,
                                  int64_t *position, void *outbuf,
                                  int64_t outcount, int variant, void *stream) {
-  if (!sb || !position || outcount < 0 || *position < 0) {
+  if (!sb || !position || outcount < 0 || *position < 0 || insize < 0) {
     return set_error(TEMPI_ERR_ARG, "bad argument");
   }
-  const int64_t packed = outcount * sb_size(*sb);
-  if (*position + packed > insize) {
+  int64_t packed = 0;
+  int err = packed_bytes(*sb, outcount, &packed);
+  if (err) {
+    return err;
+  }
+  if (packed > insize - *position) {
     return set_error(TEMPI_ERR_TRUNCATE, "need %lld bytes, have %lld",
                      (long long)(*position + packed), (long long)insize);
   }
-  int err = ensure_runtime();
+  err = ensure_runtime();
   if (err) {
     return err;
   }

@@ -108,6 +108,13 @@ int make_geom(const tempi_strided_block &sb, int64_t count, Geom *g) {
       return TEMPI_SUCCESS; // nothing to move
     }
   }
+  // the packed size bounds every product of counts formed below
+  int64_t packed = 1;
+  for (const Dim &x : dims) {
+    if (__builtin_mul_overflow(packed, x.count, &packed)) {
+      return set_error(TEMPI_ERR_ARG, "packed size overflows 64 bits");
+    }
+  }
   normalize_dims(dims);
   if ((int)dims.size() - 1 > kMaxRowDims) {
     return set_error(TEMPI_ERR_UNSUPPORTED, "%d row dimensions",

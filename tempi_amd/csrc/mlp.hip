@@ -650,8 +650,12 @@ TEMPI_EXPORT int tempi_mlp_features(const tempi_strided_block *sb,
   if (!sb || !features) {
     return set_error(TEMPI_ERR_ARG, "null argument");
   }
+  int err = check_direction(direction);
+  if (err) {
+    return err;
+  }
   Geom g;
-  int err = make_geom(*sb, incount, &g);
+  err = make_geom(*sb, incount, &g);
   if (err) {
     return err;
   }

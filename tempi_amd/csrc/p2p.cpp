@@ -530,8 +530,6 @@ int post_pending_sends() {
   return first_err;
 }
 
-// does this call take the device path?  fills the record and the device
-// address of the user buffer
 // modelled times of a contiguous transfer staged through the host and sent
 // directly (reference: SendRecv1DStaged::model and SendRecvFallback::model,
 // sender.cpp:26-61)
@@ -549,6 +547,8 @@ bool staged_beats_direct(int64_t bytes) {
   return staged < send;
 }
 
+// does this call take the device path?  fills the record and the device
+// address of the user buffer, and the method when the path decides it
 bool device_path(const void *buf, int count, MPI_Datatype dt, int peer,
                  Canon *cn, void **dev, int *method_out) {
   if (env().disable || count <= 0 || peer == MPI_PROC_NULL || !buf ||

@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 namespace tempi {
 
 namespace {
@@ -17,7 +19,7 @@ struct Runtime {
   std::once_flag probe_once;
   int device_count = 0;
   std::mutex mu;
-  bool ready = false;
+  std::atomic<bool> ready{false};
   hipStream_t stream = nullptr;
   int cus = 0;
 };
@@ -42,7 +44,7 @@ bool have_device() {
 
 int ensure_runtime() {
   Runtime &r = rt();
-  if (r.ready) {
+  if (r.ready.load(std::memory_order_acquire)) {
     return TEMPI_SUCCESS;
   }
   if (!have_device()) {
@@ -61,7 +63,7 @@ int ensure_runtime() {
   // streams, as the reference's kernStream
   TEMPI_HIP_CHECK(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
   TEMPI_LOG(LOG_INFO, "device %d (%s), %d CUs", dev, prop.gcnArchName, r.cus);
-  r.ready = true;
+  r.ready.store(true, std::memory_order_release);
   return TEMPI_SUCCESS;
 }
 
@@ -107,11 +109,11 @@ int device_address(const void *p, void **dev) {
 void finalize_runtime() {
   Runtime &r = rt();
   std::lock_guard<std::mutex> lk(r.mu);
-  if (r.ready) {
+  if (r.ready.load(std::memory_order_acquire)) {
     (void)hipStreamSynchronize(r.stream);
     (void)hipStreamDestroy(r.stream);
     r.stream = nullptr;
-    r.ready = false;
+    r.ready.store(false, std::memory_order_release);
   }
 }
 

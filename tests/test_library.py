@@ -104,6 +104,29 @@ def test_argument_errors():
     # truncation is detected before any device work
     assert L.tempi_pack_sb(ctypes.byref(sb), None, 2, None, 15, ctypes.byref(pos), -1,
                            None) == ta.TEMPI_ERR_TRUNCATE
+    # descriptors are validated before their size is taken: a dimension count
+    # beyond the arrays, and counts whose product overflows 64 bits
+    wide = ta.StridedBlock.make(0, 8, [4, 2], [1, 8])
+    wide.ndims = 99
+    huge = ta.StridedBlock.make(0, 1 << 40, [1 << 40, 1 << 40], [1, 1 << 40])
+    for d in (wide, huge):
+        assert L.tempi_pack_sb(ctypes.byref(d), None, 1, None, 8, ctypes.byref(pos), -1,
+                               None) == ta.TEMPI_ERR_ARG
+        assert L.tempi_unpack_sb(ctypes.byref(d), None, 8, ctypes.byref(pos), None, 1, -1,
+                                 None) == ta.TEMPI_ERR_ARG
+    assert L.tempi_pack_sb(ctypes.byref(sb), None, 1 << 62, None, 8, ctypes.byref(pos), -1,
+                           None) == ta.TEMPI_ERR_ARG
+    assert pos.value == 0
+    # a direction other than pack (0) or unpack (1) is refused, not guessed
+    v = ctypes.c_int()
+    assert L.tempi_select_variant(ctypes.byref(sb), 1, None, None, 7, 0,
+                                  ctypes.byref(v)) == ta.TEMPI_ERR_ARG
+    feats = (ctypes.c_float * L.tempi_mlp_num_features())()
+    assert L.tempi_mlp_features(ctypes.byref(sb), 1, None, None, -1, feats) == ta.TEMPI_ERR_ARG
+    for direction, count, code in ((7, 1, ta.TEMPI_ERR_ARG), (ta.DIR_PACK, -1, ta.TEMPI_ERR_ARG)):
+        with pytest.raises(ta.TempiError) as e:
+            ta.Batch([(sb, 0, 0, count, direction)])
+        assert e.value.code == code
     bad = ta.StridedBlock.make(0, 8, [4, 2], [2, 8])
     v = ctypes.c_int()
     assert L.tempi_select_variant(ctypes.byref(bad), 1, None, None, 0, 0,
