@@ -21,7 +21,8 @@ extras   the same faces as six separate kernels (the MPI_Pack kernels) and
 cpu_baseline  the oracle's single-threaded C gather/scatter (oracle/oracle.c)
          on the host copy of the grid, over a bounded sample; the underlying
          MPI library's own MPI_Pack / MPI_Unpack (where the reference sends
-         host buffers, src/pack.cpp:51-54) is reported beside it.
+         host buffers, src/pack.cpp:51-54) is reported beside it, and the
+         oracle's gather/scatter in one process per host core (at most 16).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -32,6 +33,7 @@ import json
 import math
 import os
 import random
+import subprocess
 import sys
 import time
 
@@ -77,7 +79,11 @@ def main():
                     help="only the 256 MiB 2D packs/unpacks (64 B - 4 KiB blocks, aligned "
                          "and 4 B off), three cold launches each, then the 256 MiB "
                          "contiguous calibration copy (for rocprofv3)")
+    ap.add_argument("--cpu-worker", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(args.cpu_worker)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share of the pack path; --gpus {args.gpus} is not "
               "supported: packing does not communicate, so N GPUs run N independent copies "
@@ -282,9 +288,10 @@ def main():
         with open(tpath) as fh:
             traffic = json.load(fh).get("per_launch")
 
-    cpu = cpu_lib = None
+    cpu = cpu_lib = cpu_all = None
     if not args.no_cpu:
         cpu, cpu_lib = cpu_baselines(args, m, orc, faces, host, np)
+        cpu_all = cpu_all_cores(faces, host.size, args.cpu_seconds / 2)
 
     # each batch launch moves half the step's packed bytes, read once and
     # written once: 2 x (packed_per_step / 2) algorithmic HBM bytes
@@ -326,6 +333,7 @@ def main():
         "extras": extras,
         "cpu_baseline": cpu,
         "cpu_mpi_library": cpu_lib,
+        "cpu_all_cores": cpu_all,
     }
     print(json.dumps(result))
     for f in faces:
@@ -790,6 +798,7 @@ def cpu_baselines(args, m, orc, faces, host, np):
         reps += 1
     el = time.perf_counter() - t0
     cpu = {"value": round(moved / el / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+           "cpu": host_cpu_model(),
            "sample": f"oracle sb_pack + sb_unpack of the 3 faces of the 518^3 float32 grid, "
                      f"{reps} steps in {el:.1f} s on 1 host thread"}
     log(f"cpu baseline: MPI library MPI_Pack + MPI_Unpack, {args.cpu_seconds / 2:.0f} s sample")
@@ -808,6 +817,76 @@ def cpu_baselines(args, m, orc, faces, host, np):
            "library": "MPICH 3.3.2 MPI_Pack / MPI_Unpack on host buffers",
            "sample": f"{reps} steps of the 3 faces in {el:.1f} s"}
     return cpu, lib
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_worker(spec):
+    """One process of the all-cores CPU baseline: builds its own copy of the
+    grid, reports ready, waits for the go line on stdin, then runs the
+    oracle's gather + scatter of the faces for the given time and prints the
+    bytes it moved.  Uses no GPU."""
+    from oracle import oracle as orc
+
+    job = json.loads(spec)
+    # the bytes' values do not change a gather's cost: counter-based fill
+    host = orc.splitmix_bytes(job["grid_bytes"], 2)
+    descs = [orc.sb_desc(*sb) for sb in job["faces"]]
+    print("ready", flush=True)
+    sys.stdin.readline()
+    moved = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < job["seconds"]:
+        for d, n in zip(descs, job["sizes"]):
+            orc.sb_unpack(d, orc.sb_pack(d, host, 0, 1), host, 0, 1)
+            moved += 2 * n
+    print(json.dumps({"moved": moved, "seconds": time.perf_counter() - t0}), flush=True)
+
+
+def cpu_all_cores(faces, grid_bytes, seconds):
+    """The same oracle gather/scatter in one single-threaded process per core
+    of this host's CPU share (at most 16), all timed over the same window;
+    the sum of their rates."""
+    nproc = min(16, len(os.sched_getaffinity(0)))
+    log(f"cpu baseline: oracle gather/scatter in {nproc} processes, {seconds:.0f} s sample")
+    job = json.dumps({"grid_bytes": int(grid_bytes), "seconds": seconds,
+                      "faces": [list(f["sb"].as_tuple()) for f in faces],
+                      "sizes": [f["size"] for f in faces]})
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", job],
+                              stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
+             for _ in range(nproc)]
+    try:
+        for p in procs:
+            if p.stdout.readline().strip() != "ready":
+                raise RuntimeError("a CPU baseline worker did not start")
+        for p in procs:
+            p.stdin.write("go\n")
+            p.stdin.flush()
+        rates = []
+        for p in procs:
+            r = json.loads(p.stdout.readline())
+            rates.append(r["moved"] / r["seconds"])
+        for p in procs:
+            p.wait(timeout=60)
+    finally:
+        for p in procs:
+            if p.poll() is None:
+                p.kill()
+                p.wait()
+    return {"value": round(sum(rates) / 1e9, 3), "unit": "GB/s", "cores": nproc,
+            "kind": "port", "cpu": host_cpu_model(),
+            "sample": f"oracle sb_pack + sb_unpack of the 3 faces in {nproc} concurrent "
+                      f"single-threaded processes, each on its own copy of the grid, "
+                      f"{seconds:.0f} s window"}
 
 
 if __name__ == "__main__":
