@@ -79,6 +79,9 @@ def main():
                     help="only the 256 MiB 2D packs/unpacks (64 B - 4 KiB blocks, aligned "
                          "and 4 B off), three cold launches each, then the 256 MiB "
                          "contiguous calibration copy (for rocprofv3)")
+    ap.add_argument("--profile-halo", action="store_true",
+                    help="only the halo batch (1664 requests), warm and cold launches, then "
+                         "the 256 MiB contiguous calibration copy (for rocprofv3)")
     ap.add_argument("--cpu-worker", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_worker:
@@ -101,6 +104,9 @@ def main():
     stream = ta.stream()
     if args.profile_large_blocks:
         profile_large_blocks(ta, stream)
+        return
+    if args.profile_halo:
+        profile_halo(ta, m, tc, stream)
         return
     n, g = 512, 3
     full = n + 2 * g
@@ -207,6 +213,28 @@ def main():
                                 ta.SELECT_DEFAULT),
                             "hbm_gbs": round(2 * f["size"] / us / 1e3, 1)})
     extras["single_kernels_warm"] = kernels
+
+    # each headline launch between its own pair of events, warm: median and
+    # trimean (Q1 + 2 median + Q3) / 4, the reference's summary statistic
+    # (src/internal/statistics.cpp:30-39)
+    per = {"pack_batch": [], "unpack_batch": []}
+    evs = [ta.Event() for _ in range(3)]
+    for _ in range(max(args.steps, 100)):
+        evs[0].record(stream)
+        pack_batch.launch(stream)
+        evs[1].record(stream)
+        unpack_batch.launch(stream)
+        evs[2].record(stream)
+        ta.synchronize(stream)
+        per["pack_batch"].append(evs[0].elapsed_ms(evs[1]) * 1e3)
+        per["unpack_batch"].append(evs[1].elapsed_ms(evs[2]) * 1e3)
+    stats = {}
+    for k, v in per.items():
+        q1, med, q3 = (float(x) for x in np.percentile(v, [25, 50, 75]))
+        stats[k] = {"n": len(v), "median_us": round(med, 3),
+                    "trimean_us": round((q1 + 2 * med + q3) / 4, 3),
+                    "min_us": round(float(np.min(v)), 3)}
+    extras["headline_launch_events"] = stats
 
     # cold caches: a read-only 1 GiB sweep before each step leaves L2 and the
     # Infinity Cache full of clean, unrelated lines
@@ -438,9 +466,10 @@ def bench_large_blocks(ta, stream):
     return res
 
 
-def bench_halos(ta, m, tc, stream):
-    """26 halo subarrays (radius 3, float64) of 64 subdomains of 128^3:
-    one batched launch / graph replay vs one tempi_pack_sb + sync per halo."""
+def halo_requests(ta, m, tc):
+    """The pack requests of 26 halo subarrays (radius 3, float64) of 64
+    subdomains of 128^3, into one packed buffer: (requests, total packed
+    bytes, buffers to keep alive)."""
     nsub, lcr, radius, quant = 64, (128, 128, 128), 3, 8
     sub_bytes = (lcr[0] + 6) * (lcr[1] + 6) * (lcr[2] + 6) * quant
     subs = [ta.DeviceBuffer(sub_bytes) for _ in range(nsub)]
@@ -459,6 +488,46 @@ def bench_halos(ta, m, tc, stream):
         for sb, size in halos:
             reqs.append((sb, s.addr(), packed.addr(off), 1, ta.DIR_PACK))
             off += size
+    return reqs, total, subs + [packed]
+
+
+def profile_halo(ta, m, tc, stream, warm=5, cold=3):
+    """The halo batch alone (for rocprofv3): `warm` launches back to back,
+    then `cold` launches each after a read-only 1 GiB sweep, then one 256 MiB
+    contiguous copy (variant 1, the FETCH_SIZE / WRITE_SIZE calibration).
+    The batch's creation launches nothing (its requests are below the size
+    at which the library times candidates), so the k_batch dispatches of a
+    trace are these launches, in this order; one JSON line each."""
+    reqs, total, keep = halo_requests(ta, m, tc)
+    b = ta.Batch(reqs)
+    ev = [ta.Event(), ta.Event()]
+    for i in range(warm + cold):
+        kind = "warm" if i < warm else "cold"
+        if kind == "cold":
+            ta.flush_caches()
+        ta.synchronize(stream)
+        ev[0].record(stream)
+        b.launch(stream)
+        ev[1].record(stream)
+        ta.synchronize(stream)
+        print(json.dumps({"launch": kind, "us": round(ev[0].elapsed_ms(ev[1]) * 1e3, 2),
+                          "requests": len(reqs), "packed_bytes": total}), flush=True)
+    b.destroy()
+    cal = 256 << 20
+    a, c = ta.DeviceBuffer(cal), ta.DeviceBuffer(cal)
+    a.fill(1)
+    ta.pack_sb(ta.StridedBlock.make(0, cal, [cal], [1]), a.addr(), 1, c.addr(), cal, 0, 1,
+               stream)
+    ta.synchronize(stream)
+    for x in keep + [a, c]:
+        x.free()
+
+
+def bench_halos(ta, m, tc, stream):
+    """26 halo subarrays (radius 3, float64) of 64 subdomains of 128^3:
+    one batched launch / graph replay vs one tempi_pack_sb + sync per halo."""
+    reqs, total, keep = halo_requests(ta, m, tc)
+    packed = keep[-1]
     b = ta.Batch(reqs)
     nreq = len(reqs)
     for _ in range(5):
