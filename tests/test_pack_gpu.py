@@ -277,6 +277,62 @@ def test_default_policy_settles_large_copies():
     assert ta.counters()["verified_choices"] == before + 2
 
 
+BENCH_PACK_TARGETS = (64, 256, 1024, 4096, 16384, 65536, 262144, 1 << 20, 4 << 20)
+BENCH_PACK_CONTIGS = (1, 2, 4, 8, 12, 16, 20, 24, 32, 64, 128, 256)
+
+
+def test_reference_pack_benchmark_grid(mpi):
+    """The shapes of the reference's pack benchmark (bin/bench_pack.cpp:132-134,
+    :150-165): byte vectors of target / contig blocks of contig bytes at a
+    stride of 512 B, the block capped at the target, packed and unpacked
+    through MPI_Pack / MPI_Unpack with the packed side in device memory and in
+    mapped host memory (the benchmark's `stage` axis, ONE_SHOT); every image
+    against the oracle, gap bytes untouched.  Targets of 1 MiB and more take
+    the blocks of 16 B and up (their shorter blocks span up to 2 GiB)."""
+    shapes = []
+    for target in BENCH_PACK_TARGETS:
+        for contig in BENCH_PACK_CONTIGS:
+            contig = min(contig, target)
+            if target >= 1 << 20 and contig < 16:
+                continue
+            if (target // contig, contig) not in shapes:
+                shapes.append((target // contig, contig))
+    assert len(shapes) > 80
+    for k, (nb, bl) in enumerate(shapes):
+        spec = tc.make_2d_byte_vector(nb, bl, 512)
+        o = orc.OracleType(spec)
+        ext = o.extent()
+        host = orc.splitmix_bytes(ext, 3000 + k)
+        expected = o.pack(host, 0, 1)
+        ref = np.full(ext, 0xAB, np.uint8)
+        o.unpack(expected, ref, 0, 1)
+        h = mpi.type_from_spec(spec)
+        src = ta.DeviceBuffer(ext)
+        src.upload(host)
+        for mapped in (False, True):
+            where = f"{nb} x {bl} B, packed in {'mapped host' if mapped else 'device'} memory"
+            packed = ta.DeviceBuffer(expected.size + 64, mapped=mapped)
+            packed.fill(0x5A)
+            before = ta.counters()
+            err, pos = mpi.pack(src.addr(), 1, h, packed.addr(), expected.size)
+            assert err == 0 and pos == expected.size, where
+            got = packed.download()
+            assert (got[:expected.size] == expected).all(), f"pack mismatch, {where}"
+            assert (got[expected.size:] == 0x5A).all(), f"pack overran, {where}"
+            dst = ta.DeviceBuffer(ext)
+            dst.fill(0xAB)
+            err, pos = mpi.unpack(packed.addr(), expected.size, dst.addr(), 1, h)
+            assert err == 0 and pos == expected.size, where
+            assert (dst.download() == ref).all(), f"unpack mismatch, {where}"
+            after = ta.counters()
+            assert after["library_pack_calls"] == before["library_pack_calls"], where
+            assert after["library_unpack_calls"] == before["library_unpack_calls"], where
+            dst.free()
+            packed.free()
+        src.free()
+        mpi.free(h)
+
+
 def test_one_shot_into_mapped_host_memory(mpi):
     """MPI_Pack from device memory straight into pinned mapped host memory
     (the ONE_SHOT method) and MPI_Unpack from it."""
