@@ -303,6 +303,7 @@ def main():
     if not args.no_extras:
         extras["large_blocks_cold"] = bench_large_blocks(ta, stream)
         extras["halo_batch"] = bench_halos(ta, m, tc, stream)
+        extras["scattered_rows_cold"] = bench_scattered_rows(ta, stream)
         extras["random_requests"] = bench_random_requests(ta, stream)
         extras["vector_2d"] = bench_vector(ta, m, tc, stream)
         extras["system_tables"] = bench_system_tables(ta)
@@ -572,6 +573,57 @@ def bench_halos(ta, m, tc, stream):
                                                      / res["per_call_async_packs_per_s"], 1)
     log(f"halo batch: {res}")
     b.destroy()
+    return res
+
+
+def bench_scattered_rows(ta, stream, reps=5):
+    """2^20 rows of 24 B at a 1072-B stride (the pattern of the halo's x
+    faces, 1.1 GB span), packed cold (caches swept before each launch) by
+    the single-copy kernel, as a one-request batch and as 64 requests of one
+    batch: the line-read rate of the pattern in each form (HIP events around
+    each launch, median of `reps`)."""
+    n, row, stride = 1 << 20, 24, 1072
+    span = (n - 1) * stride + row
+    src = ta.DeviceBuffer(span)
+    src.fill(3)
+    dst = ta.DeviceBuffer(n * row)
+    sb = ta.StridedBlock.make(0, span, [row, n], [1, stride])
+    ev = [ta.Event(), ta.Event()]
+
+    def cold(fn):
+        ts = []
+        for _ in range(reps):
+            ta.flush_caches()
+            ta.synchronize(stream)
+            ev[0].record(stream)
+            fn()
+            ev[1].record(stream)
+            ta.synchronize(stream)
+            ts.append(ev[0].elapsed_ms(ev[1]) * 1e3)
+        return sorted(ts)[len(ts) // 2]
+
+    v = ta.select_variant(sb, 1, src.addr(), dst.addr(), ta.DIR_PACK, ta.SELECT_DEFAULT)
+    one = ta.Batch([(sb, src.addr(), dst.addr(), 1, ta.DIR_PACK)])
+    part = n // 64
+    sbk = ta.StridedBlock.make(0, (part - 1) * stride + row, [row, part], [1, stride])
+    many = ta.Batch([(sbk, src.addr(k * part * stride), dst.addr(k * part * row), 1,
+                      ta.DIR_PACK) for k in range(64)])
+    res = {}
+    for name, fn in (
+            ("single_copy_kernel",
+             lambda: ta.pack_sb(sb, src.addr(), 1, dst.addr(), n * row, 0, v, stream)),
+            ("batch_of_1", lambda: one.launch(stream)),
+            ("batch_of_64", lambda: many.launch(stream))):
+        us = cold(fn)
+        res[name] = {"us": round(us, 2), "line_reads_per_s": round(n / (us * 1e-6), 1),
+                     "line_tbs": round(n * 128 / us / 1e6, 2)}
+    ok = bool((dst.download() == 3).all())
+    one.destroy()
+    many.destroy()
+    src.free()
+    dst.free()
+    res["packed_ok"] = ok
+    log(f"scattered 24-B rows (cold): {res}")
     return res
 
 
