@@ -7,6 +7,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <atomic>
+#include <charconv>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -33,6 +34,21 @@ void log_message(int level, const char *fmt, ...)
 // record a message for tempi_last_error() and return `code`
 int set_error(int code, const char *fmt, ...)
     __attribute__((format(printf, 2, 3)));
+
+// Numbers in the files this library reads and writes (perf.json, model
+// weights) are converted with std::from_chars / std::to_chars, which ignore
+// the process locale: an application that calls setlocale() with a decimal
+// comma neither breaks the files nor has them misread.
+template <class T> bool parse_number(const char *b, const char *e, T *out) {
+  const std::from_chars_result r = std::from_chars(b, e, *out);
+  return r.ec == std::errc() && r.ptr == e;
+}
+// shortest form that reads back to the same value; "" when it does not fit
+template <class T> std::string format_number(T v) {
+  char buf[64];
+  const std::to_chars_result r = std::to_chars(buf, buf + sizeof buf, v);
+  return r.ec == std::errc() ? std::string(buf, r.ptr) : std::string();
+}
 
 // TEMPI_DIR_PACK or TEMPI_DIR_UNPACK; any other value is an argument error
 // (it would otherwise select a copy direction silently)

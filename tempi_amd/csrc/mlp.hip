@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -418,23 +419,53 @@ static int load_model_file(Model &m, const std::string &path) {
   if (!f) {
     return set_error(TEMPI_ERR_IO, "cannot open model %s", path.c_str());
   }
-  char magic[32] = {0};
-  int version = 0, in = 0, h1 = 0, h2 = 0, out = 0;
-  bool ok = fscanf(f, "%31s %d", magic, &version) == 2 &&
-            0 == strcmp(magic, "tempi-mlp") && version == 1 &&
-            fscanf(f, " in %d h1 %d h2 %d out %d", &in, &h1, &h2, &out) == 4 &&
-            in == kIn && h1 == kH1 && h2 == kH2 && out == kOut;
-  std::vector<float> norm(2 * kIn), params(kNParams);
-  char tag[32];
-  ok = ok && fscanf(f, "%31s", tag) == 1 && 0 == strcmp(tag, "norm");
-  for (int i = 0; ok && i < 2 * kIn; ++i) {
-    ok = fscanf(f, "%f", &norm[i]) == 1 && std::isfinite(norm[i]);
-  }
-  ok = ok && fscanf(f, "%31s", tag) == 1 && 0 == strcmp(tag, "params");
-  for (int i = 0; ok && i < kNParams; ++i) {
-    ok = fscanf(f, "%f", &params[i]) == 1 && std::isfinite(params[i]);
+  std::string text;
+  char chunk[4096];
+  size_t got = 0;
+  while ((got = fread(chunk, 1, sizeof chunk, f)) > 0) {
+    text.append(chunk, got);
   }
   fclose(f);
+  // whitespace-separated tokens; numbers read without the process locale
+  std::vector<std::string> tok;
+  for (size_t i = 0; i < text.size();) {
+    while (i < text.size() && isspace((unsigned char)text[i])) {
+      ++i;
+    }
+    size_t j = i;
+    while (j < text.size() && !isspace((unsigned char)text[j])) {
+      ++j;
+    }
+    if (j > i) {
+      tok.emplace_back(text, i, j - i);
+    }
+    i = j;
+  }
+  size_t at = 0;
+  auto word = [&](const char *w) { return at < tok.size() && tok[at++] == w; };
+  auto integer = [&](int want) {
+    int v = 0;
+    return at < tok.size() &&
+           parse_number(tok[at].data(), tok[at].data() + tok[at].size(), &v) &&
+           ++at && v == want;
+  };
+  auto real = [&](float *v) {
+    return at < tok.size() &&
+           parse_number(tok[at].data(), tok[at].data() + tok[at].size(), v) &&
+           ++at && std::isfinite(*v);
+  };
+  bool ok = word("tempi-mlp") && integer(1) && word("in") && integer(kIn) &&
+            word("h1") && integer(kH1) && word("h2") && integer(kH2) &&
+            word("out") && integer(kOut) && word("norm");
+  std::vector<float> norm(2 * kIn), params(kNParams);
+  for (int i = 0; ok && i < 2 * kIn; ++i) {
+    ok = real(&norm[i]);
+  }
+  ok = ok && word("params");
+  for (int i = 0; ok && i < kNParams; ++i) {
+    ok = real(&params[i]);
+  }
+  ok = ok && at == tok.size();
   if (!ok) {
     return set_error(TEMPI_ERR_IO, "malformed model file %s", path.c_str());
   }
@@ -831,11 +862,11 @@ TEMPI_EXPORT int tempi_mlp_save(const char *path, const float *params,
   fprintf(f, "tempi-mlp 1\nin %d h1 %d h2 %d out %d\nnorm", kIn, kH1, kH2,
           kOut);
   for (int i = 0; i < 2 * kIn; ++i) {
-    fprintf(f, " %.9g", norm[i]);
+    fprintf(f, " %s", format_number(norm[i]).c_str());
   }
   fprintf(f, "\nparams");
   for (int i = 0; i < kNParams; ++i) {
-    fprintf(f, "%s%.9g", (i % 16) ? " " : "\n", params[i]);
+    fprintf(f, "%s%s", (i % 16) ? " " : "\n", format_number(params[i]).c_str());
   }
   fprintf(f, "\n");
   const bool ok = 0 == fclose(f);

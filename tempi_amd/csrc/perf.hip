@@ -339,10 +339,8 @@ struct JParser {
     if (num.empty()) {
       return fail("unexpected character");
     }
-    char *e = nullptr;
     v.kind = JVal::NUM;
-    v.num = strtod(num.c_str(), &e);
-    if (!e || *e) {
+    if (!parse_number(num.data(), num.data() + num.size(), &v.num)) {
       return fail("bad number");
     }
     return true;
@@ -407,9 +405,7 @@ bool table_from(const JVal &j, Table *out) {
 
 void put_num(std::ostream &os, double v) {
   if (std::isfinite(v)) {
-    char b[40];
-    snprintf(b, sizeof b, "%.17g", v);
-    os << b;
+    os << format_number(v);
   } else {
     os << "null";
   }

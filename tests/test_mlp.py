@@ -115,6 +115,32 @@ def test_model_file_round_trip_and_rejects(tmp_path):
     trunc.write_text(open(path).read()[:2000])
     assert L.tempi_mlp_load(str(trunc).encode()) == ta.TEMPI_ERR_IO
     assert L.tempi_mlp_load(b"/nonexistent/model.txt") == ta.TEMPI_ERR_IO
+    extra = tmp_path / "extra.txt"
+    extra.write_text(open(path).read() + "0.5\n")
+    assert L.tempi_mlp_load(str(extra).encode()) == ta.TEMPI_ERR_IO
+    # seeded mutations: each file loads whole or is refused
+    good = open(path).read()
+    rng = np.random.default_rng(12)
+    refused = 0
+    for k in range(200):
+        t = list(good)
+        i = int(rng.integers(0, len(t)))
+        t[i] = "x-. 9e,+"[int(rng.integers(0, 8))]
+        if k % 3 == 0:
+            del t[int(rng.integers(0, len(t))):]
+        mut = tmp_path / "mut.txt"
+        mut.write_text("".join(t))
+        err = L.tempi_mlp_load(str(mut).encode())
+        assert err in (0, ta.TEMPI_ERR_IO), k
+        assert L.tempi_mlp_loaded() == (1 if err == 0 else 0), k
+        refused += err != 0
+    assert 0 < refused < 200
+    # the installed weights files load
+    for name, load in (("selector_mlp.txt", L.tempi_mlp_load),
+                       ("method_mlp.txt", L.tempi_method_model_load)):
+        installed = os.path.join(os.path.dirname(ta.LIB_PATH), name)
+        if os.path.exists(installed):
+            assert load(installed.encode()) == 0, name
     # without a model the selection is the closed-form rule
     sb = ta.StridedBlock.make(0, 524032, [256, 1024], [1, 512])
     assert ta.select_variant(sb, 1, 1 << 20, 1 << 20, method=ta.SELECT_MODEL) == \

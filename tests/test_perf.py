@@ -164,6 +164,46 @@ def test_malformed_files_are_rejected_and_keep_the_tables(tmp_path):
     assert ta.lib().tempi_perf_load(b"/nonexistent/perf.json") == ta.TEMPI_ERR_IO
 
 
+def test_mutated_files_load_or_fail_cleanly(tmp_path):
+    """Seeded mutations of a valid perf.json (characters replaced by JSON
+    punctuation, digits and letters; spans deleted or repeated; truncation;
+    deep nesting): each either loads whole or is refused with TEMPI_ERR_IO
+    and leaves the tables as they were."""
+    text = open(FIXTURE).read()
+    rng = np.random.default_rng(11)
+    alphabet = list('{}[],:"-+.eE0123456789 tfnul\\x')
+    cases = ["[" * 100000, "{" + '"a":' * 5000 + "1" + "}" * 5000]
+    for _ in range(300):
+        t = list(text)
+        for _ in range(int(rng.integers(1, 4))):
+            i = int(rng.integers(0, len(t)))
+            op = int(rng.integers(0, 4))
+            if op == 0:
+                t[i] = alphabet[int(rng.integers(0, len(alphabet)))]
+            elif op == 1:
+                del t[i:i + int(rng.integers(1, 40))]
+            elif op == 2:
+                t[i:i] = t[i:i + int(rng.integers(1, 40))]
+            else:
+                del t[i:]
+        cases.append("".join(t))
+    tsys.import_performance(FIXTURE)
+    keep = [tsys.get_table(k) for k in range(4)]
+    loaded = 0
+    path = tmp_path / "perf.json"
+    for k, case in enumerate(cases):
+        path.write_text(case)
+        err = ta.lib().tempi_perf_load(str(path).encode())
+        assert err in (ta.TEMPI_SUCCESS, ta.TEMPI_ERR_IO), k
+        if err == ta.TEMPI_SUCCESS:
+            loaded += 1
+            assert all(tsys.get_table(t).shape == (9, 9) for t in range(4)), k
+            tsys.import_performance(FIXTURE)
+        else:
+            assert all((tsys.get_table(t) == keep[t]).all() for t in range(4)), k
+    assert 0 < loaded < len(cases)
+
+
 def test_models_and_choice_match_the_oracle():
     tsys.import_performance(FIXTURE)
     ref = _fixture_times()
