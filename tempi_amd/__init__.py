@@ -113,6 +113,7 @@ class Counters(ctypes.Structure):
             "p2p_library",
             "trial_launches",
             "method_model_choices",
+            "host_staged_calls",
         )
     ]
 

@@ -214,6 +214,7 @@ TEMPI_EXPORT int tempi_counters_get(tempi_counters *out) {
   out->p2p_library = c.p2p_library;
   out->trial_launches = c.trial_launches;
   out->method_model_choices = c.method_model_choices;
+  out->host_staged_calls = c.host_staged_calls;
   return TEMPI_SUCCESS;
 }
 
@@ -236,6 +237,7 @@ TEMPI_EXPORT int tempi_counters_reset(void) {
   c.p2p_library = 0;
   c.trial_launches = 0;
   c.method_model_choices = 0;
+  c.host_staged_calls = 0;
   return TEMPI_SUCCESS;
 }
 

@@ -111,6 +111,7 @@ struct Counters {
   std::atomic<int64_t> p2p_library{0}; // forwarded to the MPI library
   std::atomic<int64_t> trial_launches{0}; // copies timed by the library
   std::atomic<int64_t> method_model_choices{0}; // staging methods the MLP chose
+  std::atomic<int64_t> host_staged_calls{0}; // pack/unpack staged for pageable memory
 };
 Counters &counters();
 

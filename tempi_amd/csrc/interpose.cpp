@@ -69,6 +69,49 @@ int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
   return launch_copy(g, strided, packed, dir, c.variant, stream, c.max_blocks);
 }
 
+// A device staging buffer from the pool, returned on scope exit.
+struct DeviceStage {
+  void *p = nullptr;
+  ~DeviceStage() { pool_release(false, p); }
+};
+
+// MPI_Pack / MPI_Unpack on device-accessible buffers.  `strided` is the
+// device address of the strided buffer; `packed` is the first packed byte,
+// either a device address or, with `host_packed`, pageable host memory the
+// device cannot address: then the kernel works on a device staging buffer
+// and the packed bytes are copied across.
+int pack_or_unpack(const Canon &cn, int count, const char *strided,
+                   char *packed, bool host_packed, int dir) {
+  hipStream_t s = library_stream();
+  const int64_t bytes = (int64_t)count * cn.size;
+  DeviceStage stage;
+  char *kernel_packed = packed;
+  if (host_packed) {
+    stage.p = pool_acquire(false, (size_t)std::max<int64_t>(bytes, 1));
+    if (!stage.p) {
+      return TEMPI_ERR_NOMEM;
+    }
+    kernel_packed = static_cast<char *>(stage.p);
+    ++counters().host_staged_calls;
+    if (dir == TEMPI_DIR_UNPACK &&
+        hipSuccess != hipMemcpyAsync(kernel_packed, packed, (size_t)bytes,
+                                     hipMemcpyHostToDevice, s)) {
+      return set_error(TEMPI_ERR_HIP, "staging copy failed");
+    }
+  }
+  int err = run_copy(cn.sb, count, strided, kernel_packed, dir, -1, s);
+  if (!err && host_packed && dir == TEMPI_DIR_PACK &&
+      hipSuccess != hipMemcpyAsync(packed, kernel_packed, (size_t)bytes,
+                                   hipMemcpyDeviceToHost, s)) {
+    err = set_error(TEMPI_ERR_HIP, "staging copy failed");
+  }
+  // the staging buffer goes back to the pool only after the stream is done
+  if (hipSuccess != hipStreamSynchronize(s) && !err) {
+    err = set_error(TEMPI_ERR_HIP, "stream synchronize failed");
+  }
+  return err;
+}
+
 } // namespace tempi
 
 using namespace tempi;
@@ -170,12 +213,16 @@ TEMPI_EXPORT int MPI_Pack(const void *inbuf, int incount, MPI_Datatype datatype,
   const RealMpi &lib = real_mpi();
   Canon cn;
   void *din = nullptr, *dout = nullptr;
+  // device data packed into memory the device cannot address (pageable host
+  // memory) is staged on the device; the reference hands that case to the
+  // MPI library, which cannot read device memory
   if (env().disable || env().no_pack || incount <= 0 || !position ||
-      !lookup(datatype, &cn) || !cn.ok ||
-      !device_address(inbuf, &din) || !device_address(outbuf, &dout)) {
+      !outbuf || !lookup(datatype, &cn) || !cn.ok ||
+      !device_address(inbuf, &din)) {
     ++counters().library_pack_calls;
     return lib.Pack(inbuf, incount, datatype, outbuf, outsize, position, comm);
   }
+  const bool host_packed = !device_address(outbuf, &dout);
   const int64_t packed = (int64_t)incount * cn.size;
   if (*position < 0 || (int64_t)*position + packed > (int64_t)outsize) {
     // the reference writes past the end here (pack.cpp:56)
@@ -186,13 +233,10 @@ TEMPI_EXPORT int MPI_Pack(const void *inbuf, int incount, MPI_Datatype datatype,
     ++counters().library_pack_calls;
     return lib.Pack(inbuf, incount, datatype, outbuf, outsize, position, comm);
   }
-  hipStream_t s = library_stream();
-  int err = run_copy(cn.sb, incount, static_cast<const char *>(din),
-                     static_cast<char *>(dout) + *position, TEMPI_DIR_PACK,
-                     -1, s);
-  if (err == TEMPI_SUCCESS && hipSuccess != hipStreamSynchronize(s)) {
-    err = set_error(TEMPI_ERR_HIP, "stream synchronize failed");
-  }
+  char *packed_at =
+      static_cast<char *>(host_packed ? outbuf : dout) + *position;
+  const int err = pack_or_unpack(cn, incount, static_cast<const char *>(din),
+                                 packed_at, host_packed, TEMPI_DIR_PACK);
   if (err) {
     TEMPI_LOG(LOG_ERROR, "MPI_Pack: %s", tempi_last_error());
     MPI_Comm_call_errhandler(comm, MPI_ERR_INTERN);
@@ -209,13 +253,16 @@ TEMPI_EXPORT int MPI_Unpack(const void *inbuf, int insize, int *position,
   const RealMpi &lib = real_mpi();
   Canon cn;
   void *din = nullptr, *dout = nullptr;
+  // packed bytes in pageable host memory unpacked into device data: staged
+  // on the device, as in MPI_Pack
   if (env().disable || env().no_pack || outcount <= 0 || !position ||
-      !lookup(datatype, &cn) || !cn.ok ||
-      !device_address(inbuf, &din) || !device_address(outbuf, &dout)) {
+      !inbuf || !lookup(datatype, &cn) || !cn.ok ||
+      !device_address(outbuf, &dout)) {
     ++counters().library_unpack_calls;
     return lib.Unpack(inbuf, insize, position, outbuf, outcount, datatype,
                       comm);
   }
+  const bool host_packed = !device_address(inbuf, &din);
   const int64_t packed = (int64_t)outcount * cn.size;
   if (*position < 0 || (int64_t)*position + packed > (int64_t)insize) {
     MPI_Comm_call_errhandler(comm, MPI_ERR_TRUNCATE);
@@ -226,13 +273,11 @@ TEMPI_EXPORT int MPI_Unpack(const void *inbuf, int insize, int *position,
     return lib.Unpack(inbuf, insize, position, outbuf, outcount, datatype,
                       comm);
   }
-  hipStream_t s = library_stream();
-  int err = run_copy(cn.sb, outcount, static_cast<const char *>(dout),
-                     static_cast<char *>(din) + *position, TEMPI_DIR_UNPACK,
-                     -1, s);
-  if (err == TEMPI_SUCCESS && hipSuccess != hipStreamSynchronize(s)) {
-    err = set_error(TEMPI_ERR_HIP, "stream synchronize failed");
-  }
+  char *packed_at =
+      static_cast<char *>(host_packed ? const_cast<void *>(inbuf) : din) +
+      *position;
+  const int err = pack_or_unpack(cn, outcount, static_cast<const char *>(dout),
+                                 packed_at, host_packed, TEMPI_DIR_UNPACK);
   if (err) {
     TEMPI_LOG(LOG_ERROR, "MPI_Unpack: %s", tempi_last_error());
     MPI_Comm_call_errhandler(comm, MPI_ERR_INTERN);

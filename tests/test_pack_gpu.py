@@ -361,6 +361,43 @@ def test_one_shot_into_mapped_host_memory(mpi):
     mpi.free(h)
 
 
+def test_pageable_host_packed_buffer_is_staged(mpi):
+    """MPI_Pack of device data into ordinary host memory, and MPI_Unpack of
+    it back into device data: the kernel works on a device staging buffer
+    and the packed bytes are copied across (the reference would hand the
+    device pointer to the MPI library); position advances, bytes match the
+    oracle, a second object lands after the first."""
+    spec = tc.grid_face_subarray(32, 3, "x")
+    o = orc.OracleType(spec)
+    host = orc.splitmix_bytes(o.extent() * 2, 21)
+    src = ta.DeviceBuffer(host.size)
+    src.upload(host)
+    expected = o.pack(host, 0, 2)
+    h = mpi.type_from_spec(spec)
+    out = np.full(expected.size + 8, 0x5A, np.uint8)
+    before = ta.counters()
+    err, pos = mpi.pack(src.addr(), 2, h, out.ctypes.data, out.size, position=8)
+    assert err == 0 and pos == 8 + expected.size
+    assert (out[8:] == expected).all() and (out[:8] == 0x5A).all()
+    dst = ta.DeviceBuffer(host.size)
+    dst.fill(0xAB)
+    err, pos = mpi.unpack(out.ctypes.data, out.size, dst.addr(), 2, h, position=8)
+    assert err == 0 and pos == 8 + expected.size
+    ref = np.full(host.size, 0xAB, np.uint8)
+    o.unpack(expected, ref, 0, 2)
+    assert (dst.download() == ref).all()
+    after = ta.counters()
+    assert after["host_staged_calls"] == before["host_staged_calls"] + 2
+    assert after["library_pack_calls"] == before["library_pack_calls"]
+    assert after["library_unpack_calls"] == before["library_unpack_calls"]
+    # too small a host buffer is still a truncation error, nothing written
+    mpi.errors_return()
+    small = np.full(expected.size - 1, 0x5A, np.uint8)
+    err, _ = mpi.pack(src.addr(), 2, h, small.ctypes.data, small.size)
+    assert err == mpi.ERR_TRUNCATE and (small == 0x5A).all()
+    mpi.free(h)
+
+
 def test_truncation_is_an_error(mpi):
     mpi.errors_return()
     spec = ("vector", 16, 4, 9, tc.BYTE)
