@@ -39,9 +39,6 @@ struct BatchItem {
 
 constexpr uint32_t kXcds = 8;
 
-// requests smaller than this use the closed-form variant rule in a batch
-constexpr int64_t kBatchSelectMinBytes = 1ll << 20;
-
 // One block-pass: tile `l` of a request split into `nt` tiles moves K
 // blocks of GROUPS units, nt * GROUPS units apart -- units l*GROUPS + g +
 // k*nt*GROUPS, every unit of the request exactly once over the nt tiles.
@@ -393,16 +390,11 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
     q.dir = r.direction;
     q.variant = r.variant;
     if (q.variant < 0 || q.variant >= kNumVariants) {
-      // The autotuner and the learned selector rank variants by the time of
-      // one launch of the copy alone.  Below about a megabyte that time is
-      // launch latency, which says nothing about what the copy's tiles cost
-      // inside a shared launch; such requests take the closed-form rule,
-      // which sizes tiles for throughput.  Larger requests take the
-      // per-copy policy.
-      const int m = q.g.L * q.g.rows < kBatchSelectMinBytes
-                        ? TEMPI_SELECT_HEURISTIC
-                        : method;
-      q.variant = choose(q.g, q.ds, q.dp, q.dir, m, s).variant;
+      // the per-copy policy at every size: on MI355X the learned
+      // selector's picks for requests under 1 MiB ran batches of grid
+      // faces 5-28 % faster than the closed-form rule's (faces of 128^3 -
+      // 256^3 float grids) and the 64-subdomain halo batch as fast
+      q.variant = choose(q.g, q.ds, q.dp, q.dir, method, s).variant;
     }
     b->bytes += q.g.L * q.g.rows;
     rs.push_back(q);
