@@ -393,7 +393,7 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
       // the per-copy policy at every size: on MI355X the learned
       // selector's picks for requests under 1 MiB ran batches of grid
       // faces 5-28 % faster than the closed-form rule's (faces of 128^3 -
-      // 256^3 float grids) and the 64-subdomain halo batch as fast
+      // 256^3 float grids) and the 64-subdomain halo batch within 1-2 %
       q.variant = choose(q.g, q.ds, q.dp, q.dir, method, s).variant;
     }
     b->bytes += q.g.L * q.g.rows;
