@@ -496,9 +496,10 @@ def profile_halo(ta, m, tc, stream, warm=5, cold=3):
     """The halo batch alone (for rocprofv3): `warm` launches back to back,
     then `cold` launches each after a read-only 1 GiB sweep, then one 256 MiB
     contiguous copy (variant 1, the FETCH_SIZE / WRITE_SIZE calibration).
-    The batch's creation launches nothing (its requests are below the size
-    at which the library times candidates), so the k_batch dispatches of a
-    trace are these launches, in this order; one JSON line each."""
+    The batch's creation launches no copy kernel (its requests are below
+    the size at which the library times candidates; the variant selector's
+    inference runs as k_mlp_infer), so the k_batch dispatches of a trace are
+    these launches, in this order; one JSON line each."""
     reqs, total, keep = halo_requests(ta, m, tc)
     b = ta.Batch(reqs)
     ev = [ta.Event(), ta.Event()]
