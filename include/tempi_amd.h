@@ -233,8 +233,8 @@ int tempi_launch_latency_us(int reps, float *host_us, float *device_us);
    a captured hipGraph.  Requests in one batch must be independent (no
    request writes bytes another reads or writes).  Nothing is synchronized:
    wait on the stream.  `method` (TEMPI_SELECT_*) picks the variant of each
-   request that leaves it to the library; requests under 1 MiB always take
-   the closed-form rule. */
+   request that leaves it to the library (variant -1), as for a single copy
+   of that request. */
 typedef struct tempi_batch_request {
   tempi_strided_block sb; /* layout of one object                          */
   const void *strided;    /* strided buffer origin (pack: in, unpack: out)  */
