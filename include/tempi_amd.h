@@ -337,8 +337,9 @@ typedef struct tempi_counters {
   int64_t p2p_library;          /* point-to-point calls forwarded as they are */
   int64_t trial_launches;       /* copies the library timed (not the caller's) */
   int64_t method_model_choices; /* staging methods chosen by the method MLP */
-  int64_t host_staged_calls;    /* MPI_Pack / MPI_Unpack between device data and
-                                   pageable host memory, staged on the device */
+  int64_t host_staged_calls;    /* MPI_Pack / MPI_Unpack with one side in memory
+                                   the other side's engine cannot address,
+                                   staged through device or pinned host memory */
 } tempi_counters;
 int tempi_counters_get(tempi_counters *out);
 int tempi_counters_reset(void);

@@ -190,6 +190,9 @@ hipStream_t library_stream(); // valid after ensure_runtime()
 int device_cu_count();
 // returns 1 and the device-usable address when the kernels can read/write p
 int device_address(const void *p, void **dev);
+// true for device memory the host cannot load or store (not host-mapped,
+// not managed)
+bool device_only(const void *p);
 void finalize_runtime();
 
 // ---------------------------------------------------------------------------

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <unordered_map>
 
 namespace tempi {
@@ -108,6 +109,52 @@ int pack_or_unpack(const Canon &cn, int count, const char *strided,
   // the staging buffer goes back to the pool only after the stream is done
   if (hipSuccess != hipStreamSynchronize(s) && !err) {
     err = set_error(TEMPI_ERR_HIP, "stream synchronize failed");
+  }
+  return err;
+}
+
+// Host data and packed bytes in device memory the host cannot address: the
+// MPI library packs (or unpacks) the host side, as it does for host buffers,
+// and the packed bytes cross through pinned host memory.  Without this the
+// library would be handed a device pointer.  `bytes` packed bytes at
+// `packed`; MPI error code.
+int library_through_host(bool pack, const void *inbuf, int count,
+                         MPI_Datatype datatype, void *outbuf, int bytes,
+                         int *position, MPI_Comm comm) {
+  if (ensure_runtime()) {
+    return MPI_ERR_INTERN;
+  }
+  void *stage = pool_acquire(true, (size_t)std::max(bytes, 1));
+  if (!stage) {
+    return MPI_ERR_NO_MEM;
+  }
+  ++counters().host_staged_calls;
+  hipStream_t s = library_stream();
+  int err = MPI_SUCCESS;
+  int pos = 0;
+  if (pack) {
+    err = real_mpi().Pack(inbuf, count, datatype, stage, bytes, &pos, comm);
+    if (err == MPI_SUCCESS &&
+        (hipSuccess != hipMemcpyAsync(static_cast<char *>(outbuf) + *position,
+                                      stage, (size_t)pos,
+                                      hipMemcpyHostToDevice, s) ||
+         hipSuccess != hipStreamSynchronize(s))) {
+      err = MPI_ERR_INTERN;
+    }
+  } else {
+    if (hipSuccess != hipMemcpyAsync(stage,
+                                     static_cast<const char *>(inbuf) + *position,
+                                     (size_t)bytes, hipMemcpyDeviceToHost, s) ||
+        hipSuccess != hipStreamSynchronize(s)) {
+      err = MPI_ERR_INTERN;
+    } else {
+      err = real_mpi().Unpack(stage, bytes, &pos, outbuf, count, datatype,
+                              comm);
+    }
+  }
+  pool_release(true, stage);
+  if (err == MPI_SUCCESS) {
+    *position += pos;
   }
   return err;
 }
@@ -219,6 +266,18 @@ TEMPI_EXPORT int MPI_Pack(const void *inbuf, int incount, MPI_Datatype datatype,
   if (env().disable || env().no_pack || incount <= 0 || !position ||
       !outbuf || !lookup(datatype, &cn) || !cn.ok ||
       !device_address(inbuf, &din)) {
+    if (!env().disable && incount > 0 && position && *position >= 0 &&
+        inbuf && !device_address(inbuf, &din) && device_only(outbuf)) {
+      // host data packed into device memory
+      int size = 0;
+      if (MPI_SUCCESS != MPI_Pack_size(incount, datatype, comm, &size) ||
+          (int64_t)*position + size > (int64_t)outsize) {
+        MPI_Comm_call_errhandler(comm, MPI_ERR_TRUNCATE);
+        return MPI_ERR_TRUNCATE;
+      }
+      return library_through_host(true, inbuf, incount, datatype, outbuf, size,
+                                  position, comm);
+    }
     ++counters().library_pack_calls;
     return lib.Pack(inbuf, incount, datatype, outbuf, outsize, position, comm);
   }
@@ -258,6 +317,18 @@ TEMPI_EXPORT int MPI_Unpack(const void *inbuf, int insize, int *position,
   if (env().disable || env().no_pack || outcount <= 0 || !position ||
       !inbuf || !lookup(datatype, &cn) || !cn.ok ||
       !device_address(outbuf, &dout)) {
+    if (!env().disable && outcount > 0 && position && *position >= 0 &&
+        outbuf && !device_address(outbuf, &dout) && device_only(inbuf)) {
+      // packed bytes in device memory unpacked into host data
+      int size = 0;
+      if (MPI_SUCCESS != MPI_Pack_size(outcount, datatype, comm, &size) ||
+          (int64_t)*position + size > (int64_t)insize) {
+        MPI_Comm_call_errhandler(comm, MPI_ERR_TRUNCATE);
+        return MPI_ERR_TRUNCATE;
+      }
+      return library_through_host(false, inbuf, outcount, datatype, outbuf,
+                                  size, position, comm);
+    }
     ++counters().library_unpack_calls;
     return lib.Unpack(inbuf, insize, position, outbuf, outcount, datatype,
                       comm);

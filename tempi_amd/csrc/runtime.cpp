@@ -106,6 +106,18 @@ int device_address(const void *p, void **dev) {
   }
 }
 
+bool device_only(const void *p) {
+  if (!p || !have_device()) {
+    return false;
+  }
+  hipPointerAttribute_t a{};
+  if (hipSuccess != hipPointerGetAttributes(&a, p)) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice;
+}
+
 void finalize_runtime() {
   Runtime &r = rt();
   std::lock_guard<std::mutex> lk(r.mu);

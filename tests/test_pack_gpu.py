@@ -398,6 +398,38 @@ def test_pageable_host_packed_buffer_is_staged(mpi):
     mpi.free(h)
 
 
+def test_host_data_with_packed_bytes_in_device_memory(mpi):
+    """MPI_Pack of host data into device memory and MPI_Unpack from it into
+    host data: the MPI library packs and unpacks the host side, as it does
+    for host buffers, and the packed bytes cross through pinned host memory
+    (the reference would hand the library a device pointer)."""
+    spec = tc.grid_face_subarray(32, 3, "y")
+    o = orc.OracleType(spec)
+    host = orc.splitmix_bytes(o.extent() * 2, 23)
+    expected = o.pack(host, 0, 2)
+    h = mpi.type_from_spec(spec)
+    packed = ta.DeviceBuffer(expected.size + 16)
+    packed.fill(0x5A)
+    before = ta.counters()
+    err, pos = mpi.pack(host.ctypes.data, 2, h, packed.addr(), expected.size + 16, position=16)
+    assert err == 0 and pos == 16 + expected.size
+    got = packed.download()
+    assert (got[16:] == expected).all() and (got[:16] == 0x5A).all()
+    back = np.full(host.size, 0xAB, np.uint8)
+    err, pos = mpi.unpack(packed.addr(), expected.size + 16, back.ctypes.data, 2, h,
+                          position=16)
+    assert err == 0 and pos == 16 + expected.size
+    ref = np.full(host.size, 0xAB, np.uint8)
+    o.unpack(expected, ref, 0, 2)
+    assert (back == ref).all()
+    after = ta.counters()
+    assert after["host_staged_calls"] == before["host_staged_calls"] + 2
+    mpi.errors_return()
+    err, _ = mpi.pack(host.ctypes.data, 2, h, packed.addr(), expected.size - 1)
+    assert err == mpi.ERR_TRUNCATE
+    mpi.free(h)
+
+
 def test_truncation_is_an_error(mpi):
     mpi.errors_return()
     spec = ("vector", 16, 4, 9, tc.BYTE)
