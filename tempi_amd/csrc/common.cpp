@@ -263,7 +263,7 @@ TEMPI_EXPORT int64_t tempi_mpi_constant(const char *name, int *ok) {
       E(MPI_SUCCESS),       E(MPI_ERR_TRUNCATE),  E(MPI_ERRORS_RETURN),
       E(MPI_ERRORS_ARE_FATAL), E(MPI_THREAD_SINGLE), E(MPI_THREAD_MULTIPLE),
       E(MPI_REQUEST_NULL),  E(MPI_ANY_SOURCE),    E(MPI_ANY_TAG),
-      E(MPI_PROC_NULL),     E(MPI_ERR_COUNT),     E(MPI_INFO_NULL),
+      E(MPI_PROC_NULL),     E(MPI_ERR_COUNT),     E(MPI_ERR_BUFFER),     E(MPI_INFO_NULL),
       E(MPI_UNWEIGHTED),    E(MPI_COMM_NULL),
       {"sizeof(MPI_Status)", (int64_t)sizeof(MPI_Status)},
       {"offsetof(MPI_Status,MPI_SOURCE)", (int64_t)offsetof(MPI_Status, MPI_SOURCE)},

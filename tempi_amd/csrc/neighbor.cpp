@@ -285,6 +285,16 @@ TEMPI_EXPORT int MPI_Neighbor_alltoallw(
   void *sd = nullptr, *rd = nullptr;
   if ((outdeg > 0 && !device_address(sendbuf, &sd)) ||
       (indeg > 0 && !device_address(recvbuf, &rd)) || ensure_runtime()) {
+    if (!env().gpu_aware_mpi && ((outdeg > 0 && device_only(sendbuf)) ||
+                                 (indeg > 0 && device_only(recvbuf)))) {
+      // one side in device memory, the other not handled here: the MPI
+      // library cannot read or write device memory
+      set_error(TEMPI_ERR_UNSUPPORTED, "MPI_Neighbor_alltoallw with one buffer "
+                                       "in device memory and one in host memory");
+      TEMPI_LOG(LOG_ERROR, "%s", tempi_last_error());
+      MPI_Comm_call_errhandler(comm, MPI_ERR_BUFFER);
+      return MPI_ERR_BUFFER;
+    }
     return library();
   }
   // the cached argument set, if this call repeats one

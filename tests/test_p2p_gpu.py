@@ -247,6 +247,25 @@ def test_neighbor_alltoallw_halo_exchange_self(mpi, method):
         mpi.free(h)
 
 
+def test_neighbor_alltoallw_with_mixed_buffers_is_an_error(mpi):
+    """One buffer in device memory and the other in host memory: refused
+    with MPI_ERR_BUFFER instead of handing the MPI library (not GPU-aware
+    here) a device pointer; nothing is written."""
+    mpi.errors_return()
+    h = mpi.type_from_spec(FACE)
+    comm = mpi.dist_graph([0], [0])
+    o = orc.OracleType(FACE)
+    src = ta.DeviceBuffer(o.extent())
+    src.fill(1)
+    host_dst = np.full(o.extent(), 0xAB, np.uint8)
+    err = mpi.neighbor_alltoallw(src.addr(), [1], [0], [h], host_dst.ctypes.data, [1], [0],
+                                 [h], comm)
+    assert err == mpi.const("MPI_ERR_BUFFER")
+    assert (host_dst == 0xAB).all()
+    mpi.comm_free(comm)
+    mpi.free(h)
+
+
 def test_wildcards_proc_null_and_truncation(mpi):
     """MPI_ANY_SOURCE / MPI_ANY_TAG receives report the real source and tag;
     MPI_PROC_NULL goes to the library; a message longer than the receive is
