@@ -907,7 +907,20 @@ def bench_system_tables(ta):
 
 
 def cpu_baselines(args, m, orc, faces, host, np):
-    log(f"cpu baseline: oracle gather/scatter, {args.cpu_seconds:.0f} s sample, 1 thread")
+    """The single-core legs, pinned to one core of this process's CPU set
+    (restored afterwards)."""
+    allowed = os.sched_getaffinity(0)
+    core = min(allowed)
+    os.sched_setaffinity(0, {core})
+    try:
+        return _cpu_baselines(args, m, orc, faces, host, np, core)
+    finally:
+        os.sched_setaffinity(0, allowed)
+
+
+def _cpu_baselines(args, m, orc, faces, host, np, core):
+    log(f"cpu baseline: oracle gather/scatter, {args.cpu_seconds:.0f} s sample, 1 thread "
+        f"pinned to core {core}")
     descs = [orc.sb_desc(*f["sb"].as_tuple()) for f in faces]
     scratch = host.copy()
     moved, reps = 0, 0
@@ -922,7 +935,7 @@ def cpu_baselines(args, m, orc, faces, host, np):
     cpu = {"value": round(moved / el / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
            "cpu": host_cpu_model(),
            "sample": f"oracle sb_pack + sb_unpack of the 3 faces of the 518^3 float32 grid, "
-                     f"{reps} steps in {el:.1f} s on 1 host thread"}
+                     f"{reps} steps in {el:.1f} s on 1 host thread pinned to core {core}"}
     log(f"cpu baseline: MPI library MPI_Pack + MPI_Unpack, {args.cpu_seconds / 2:.0f} s sample")
     out = np.zeros(faces[0]["size"], np.uint8)
     moved, reps = 0, 0
@@ -937,7 +950,7 @@ def cpu_baselines(args, m, orc, faces, host, np):
     el = time.perf_counter() - t0
     lib = {"value": round(moved / el / 1e9, 3), "unit": "GB/s", "cores": 1,
            "library": "MPICH 3.3.2 MPI_Pack / MPI_Unpack on host buffers",
-           "sample": f"{reps} steps of the 3 faces in {el:.1f} s"}
+           "sample": f"{reps} steps of the 3 faces in {el:.1f} s, pinned to core {core}"}
     return cpu, lib
 
 
