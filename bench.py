@@ -707,9 +707,20 @@ def bench_vector(ta, m, tc, stream):
         ts.append(time.perf_counter() - t0)
     _, dev = timed_loop(ta, lambda: ta.pack_sb(sb, src.addr(), 1, out.addr(), size, 0, -1,
                                                stream), 200, stream)
+    # the same pack into ordinary (pageable) host memory: staged on the
+    # device, then copied across
+    pageable = np.zeros(size, np.uint8)
+    tp = []
+    for _ in range(200):
+        t0 = time.perf_counter()
+        m.pack(src.addr(), 1, h, pageable.ctypes.data, size)
+        tp.append(time.perf_counter() - t0)
+    ok = bool((pageable == 9).all())
     m.free(h)
     return {"mpi_pack_wall_us": round(float(np.median(ts)) * 1e6, 2),
             "kernel_us": round(dev * 1e6 / 200, 3),
+            "mpi_pack_into_pageable_host_wall_us": round(float(np.median(tp)) * 1e6, 2),
+            "pageable_ok": ok,
             "packed_bytes": size}
 
 

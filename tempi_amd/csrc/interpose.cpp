@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <unordered_map>
 
 namespace tempi {
@@ -70,45 +71,47 @@ int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
   return launch_copy(g, strided, packed, dir, c.variant, stream, c.max_blocks);
 }
 
-// A device staging buffer from the pool, returned on scope exit.
-struct DeviceStage {
+// A pinned, device-mapped host staging buffer from the pool, returned on
+// scope exit.
+struct HostStage {
   void *p = nullptr;
-  ~DeviceStage() { pool_release(false, p); }
+  ~HostStage() { pool_release(true, p); }
 };
 
 // MPI_Pack / MPI_Unpack on device-accessible buffers.  `strided` is the
 // device address of the strided buffer; `packed` is the first packed byte,
 // either a device address or, with `host_packed`, pageable host memory the
-// device cannot address: then the kernel works on a device staging buffer
-// and the packed bytes are copied across.
+// device cannot address: then the kernel reads or writes a pinned host
+// staging buffer through its device mapping (as ONE_SHOT does) and the host
+// copies between it and the caller's memory -- a kernel store across the
+// link plus a host memcpy, where a copy from device memory into pageable
+// memory would go through the runtime's own bounce buffer.
 int pack_or_unpack(const Canon &cn, int count, const char *strided,
                    char *packed, bool host_packed, int dir) {
   hipStream_t s = library_stream();
   const int64_t bytes = (int64_t)count * cn.size;
-  DeviceStage stage;
+  HostStage stage;
   char *kernel_packed = packed;
   if (host_packed) {
-    stage.p = pool_acquire(false, (size_t)std::max<int64_t>(bytes, 1));
-    if (!stage.p) {
+    stage.p = pool_acquire(true, (size_t)std::max<int64_t>(bytes, 1));
+    void *mapped = nullptr;
+    if (!stage.p || !device_address(stage.p, &mapped)) {
       return TEMPI_ERR_NOMEM;
     }
-    kernel_packed = static_cast<char *>(stage.p);
+    kernel_packed = static_cast<char *>(mapped);
     ++counters().host_staged_calls;
-    if (dir == TEMPI_DIR_UNPACK &&
-        hipSuccess != hipMemcpyAsync(kernel_packed, packed, (size_t)bytes,
-                                     hipMemcpyHostToDevice, s)) {
-      return set_error(TEMPI_ERR_HIP, "staging copy failed");
+    if (dir == TEMPI_DIR_UNPACK) {
+      memcpy(stage.p, packed, (size_t)bytes);
     }
   }
   int err = run_copy(cn.sb, count, strided, kernel_packed, dir, -1, s);
-  if (!err && host_packed && dir == TEMPI_DIR_PACK &&
-      hipSuccess != hipMemcpyAsync(packed, kernel_packed, (size_t)bytes,
-                                   hipMemcpyDeviceToHost, s)) {
-    err = set_error(TEMPI_ERR_HIP, "staging copy failed");
-  }
-  // the staging buffer goes back to the pool only after the stream is done
+  // the staging buffer is read (or goes back to the pool) only after the
+  // stream is done
   if (hipSuccess != hipStreamSynchronize(s) && !err) {
     err = set_error(TEMPI_ERR_HIP, "stream synchronize failed");
+  }
+  if (!err && host_packed && dir == TEMPI_DIR_PACK) {
+    memcpy(packed, stage.p, (size_t)bytes);
   }
   return err;
 }
