@@ -96,7 +96,8 @@ int pack_or_unpack(const Canon &cn, int count, const char *strided,
     stage.p = pool_acquire(true, (size_t)std::max<int64_t>(bytes, 1));
     void *mapped = nullptr;
     if (!stage.p || !device_address(stage.p, &mapped)) {
-      return TEMPI_ERR_NOMEM;
+      return set_error(TEMPI_ERR_NOMEM, "host staging buffer of %lld B",
+                       (long long)bytes);
     }
     kernel_packed = static_cast<char *>(mapped);
     ++counters().host_staged_calls;
