@@ -302,25 +302,35 @@ def main():
 
     if not args.no_extras:
         extras["large_blocks_cold"] = bench_large_blocks(ta, stream)
-        extras["halo_batch"] = bench_halos(ta, m, tc, stream)
+        extras["halo_batch"] = bench_halos(ta, m, tc, orc, stream)
         extras["scattered_rows_cold"] = bench_scattered_rows(ta, stream)
-        extras["random_requests"] = bench_random_requests(ta, stream)
+        extras["random_requests"] = bench_random_requests(ta, tc, orc, stream)
         extras["vector_2d"] = bench_vector(ta, m, tc, stream)
         extras["system_tables"] = bench_system_tables(ta)
         extras["vector_sweep"] = bench_sweep(ta, m, tc, stream)
+        extras["selector_regret"] = bench_selector_regret(ta)
         extras["isend_irecv_faces"] = bench_p2p(ta, m, faces, grid, grid2, host)
         extras["neighbor_alltoallw_halo"] = bench_neighbor(ta, m, tc)
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_grid_faces_518.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("per_launch")
+        extras["reference_mpi_pack_sweep"] = bench_reference_sweep()
+        extras["type_commit"] = bench_type_commit()
+        extras["first_call_large_copy"] = bench_first_call(ta, m, tc)
+        for k in ("halo_batch", "random_requests"):
+            bit_exact &= bool(extras[k]["bit_exact"])
 
-    cpu = cpu_lib = cpu_all = None
+    def traffic_of(name):
+        tpath = os.path.join(ROOT, "profiles", name)
+        if os.path.exists(tpath):
+            with open(tpath) as fh:
+                return json.load(fh).get("per_launch")
+        return None
+
+    traffic = traffic_of("traffic_grid_faces_518.json")
+
+    cpu = cpu_oracle = cpu_all = None
     if not args.no_cpu:
-        cpu, cpu_lib = cpu_baselines(args, m, orc, faces, host, np)
-        cpu_all = cpu_all_cores(faces, host.size, args.cpu_seconds / 2)
+        cpu, cpu_oracle = cpu_baselines(args, m, orc, faces, host, np)
+        cpu_all = cpu_all_cores(m, faces, host.size, args.cpu_seconds / 2)
 
     # each batch launch moves half the step's packed bytes, read once and
     # written once: 2 x (packed_per_step / 2) algorithmic HBM bytes
@@ -361,9 +371,23 @@ def main():
         },
         "extras": extras,
         "cpu_baseline": cpu,
-        "cpu_mpi_library": cpu_lib,
         "cpu_all_cores": cpu_all,
+        "cpu_oracle": cpu_oracle,
     }
+    halo = extras.get("halo_batch")
+    if halo:
+        # the largest batched configuration: 26 halos of 64 subdomains of
+        # 128^3 float64 in one k_batch launch, 2 x 158 183 424 algorithmic B
+        ach = 2 * halo["packed_bytes"] / (halo["batch_kernel_us"] * 1e-6) / 1e9
+        result["roofline_halo"] = {
+            "bound": "hbm", "kernel": "k_batch",
+            "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": traffic_of("traffic_halo_64x128cube_radius3.json"),
+            "algorithmic_bytes_per_launch": 2 * halo["packed_bytes"],
+            "avg_launch_us": halo["batch_kernel_us"],
+            "bit_exact": halo["bit_exact"],
+        }
     print(json.dumps(result))
     for f in faces:
         m.free(f["handle"])
@@ -467,15 +491,19 @@ def bench_large_blocks(ta, stream):
     return res
 
 
-def halo_requests(ta, m, tc):
+def halo_requests(ta, m, tc, hosts=None):
     """The pack requests of 26 halo subarrays (radius 3, float64) of 64
     subdomains of 128^3, into one packed buffer: (requests, total packed
-    bytes, buffers to keep alive)."""
+    bytes, buffers to keep alive).  With `hosts` (64 host arrays) the
+    subdomains hold those bytes, else a constant."""
     nsub, lcr, radius, quant = 64, (128, 128, 128), 3, 8
     sub_bytes = (lcr[0] + 6) * (lcr[1] + 6) * (lcr[2] + 6) * quant
     subs = [ta.DeviceBuffer(sub_bytes) for _ in range(nsub)]
-    for s in subs:
-        s.fill(7)
+    for i, s in enumerate(subs):
+        if hosts is not None:
+            s.upload(hosts[i])
+        else:
+            s.fill(7)
     halos = []
     for d in tc.halo_directions():
         h = m.type_from_spec(tc.halo_type(radius, lcr, d, quant))
@@ -525,10 +553,16 @@ def profile_halo(ta, m, tc, stream, warm=5, cold=3):
         x.free()
 
 
-def bench_halos(ta, m, tc, stream):
-    """26 halo subarrays (radius 3, float64) of 64 subdomains of 128^3:
-    one batched launch / graph replay vs one tempi_pack_sb + sync per halo."""
-    reqs, total, keep = halo_requests(ta, m, tc)
+def bench_halos(ta, m, tc, orc, stream):
+    """26 halo subarrays (radius 3, float64) of 64 subdomains of 128^3 of
+    seeded bytes: one batched launch / graph replay vs one tempi_pack_sb +
+    sync per halo; the packed bytes of the last batched launch are checked
+    against the oracle."""
+    import numpy as np
+
+    sub_bytes = 134 ** 3 * 8
+    hosts = [orc.splitmix_bytes(sub_bytes, 4000 + s) for s in range(64)]
+    reqs, total, keep = halo_requests(ta, m, tc, hosts)
     packed = keep[-1]
     b = ta.Batch(reqs)
     nreq = len(reqs)
@@ -536,6 +570,13 @@ def bench_halos(ta, m, tc, stream):
         b.launch(stream)
     reps = 50
     wall_b, dev_b = timed_loop(ta, lambda: b.launch(stream), reps, stream)
+    got = packed.download()
+    exact, off = True, 0
+    for i, r in enumerate(reqs):
+        exp = orc.sb_pack(orc.sb_desc(*r[0].as_tuple()), hosts[i // 26], 0, 1)
+        exact &= bool((got[off:off + exp.size] == exp).all())
+        off += exp.size
+    del got
     wall_g, _ = timed_loop(ta, lambda: b.graph_launch(stream), reps, stream)
 
     def per_call():
@@ -566,7 +607,9 @@ def bench_halos(ta, m, tc, stream):
         "per_call_sync_packs_per_s": round(nreq / wall_pc, 1),
         "per_call_async_packs_per_s": round(nreq / wall_pa, 1),
         "batched_us_per_launch": round(wall_b * 1e6 / reps, 2),
+        "batch_kernel_us": round(dev_b * 1e6 / reps, 2),
         "batch_kernel_hbm_gbs": round(2 * total / (dev_b / reps) / 1e9, 1),
+        "bit_exact": exact,
     }
     res["speedup_batched_vs_per_call_sync"] = round(res["batched_packs_per_s"]
                                                     / res["per_call_sync_packs_per_s"], 1)
@@ -628,37 +671,16 @@ def bench_scattered_rows(ta, stream, reps=5):
     return res
 
 
-def bench_random_requests(ta, stream, nreq=4096, seed=4):
+def bench_random_requests(ta, tc, orc, stream, nreq=4096, seed=4):
     """4096 random 1D/2D/3D strided blocks with packed sizes log-uniform in
     [64 B, 16 MiB], all reading one source buffer, packed to disjoint
     outputs: batched vs per call."""
-    rng = random.Random(seed)
-    reqs_geo = []
-    span_max = 0
-    total = 0
-    for _ in range(nreq):
-        target = int(math.exp(rng.uniform(math.log(64), math.log(16 << 20))))
-        nd = rng.choice([1, 2, 3])
-        if nd == 1:
-            counts, strides = [target], [1]
-        else:
-            bl = rng.choice([4, 8, 12, 16, 32, 64, 128, 256, 512, 1024, 2048])
-            bl = min(bl, target)
-            rows = max(1, target // bl)
-            if nd == 2:
-                counts, strides = [bl, rows], [1, bl + rng.randint(0, 512)]
-            else:
-                r1 = max(1, int(math.sqrt(rows)))
-                r2 = max(1, rows // r1)
-                s1 = bl + rng.randint(0, 512)
-                counts, strides = [bl, r1, r2], [1, s1, s1 * r1 + rng.randint(0, 4096)]
-        span = 1 + sum((c - 1) * s for c, s in zip(counts, strides)) + counts[0]
-        sb = ta.StridedBlock.make(rng.randint(0, 3) * 4, span + 16, counts, strides)
-        reqs_geo.append(sb)
-        span_max = max(span_max, span + 64)
-        total += sb.size()
+    geo, span_max = tc.random_requests(seed, nreq)
+    reqs_geo = [ta.StridedBlock.make(*g) for g in geo]
+    total = sum(sb.size() for sb in reqs_geo)
+    host = orc.splitmix_bytes(span_max, 46)
     src = ta.DeviceBuffer(span_max)
-    src.fill(3)
+    src.upload(host)
     packed = ta.DeviceBuffer(total)
     reqs, off = [], 0
     for sb in reqs_geo:
@@ -668,6 +690,13 @@ def bench_random_requests(ta, stream, nreq=4096, seed=4):
     b.launch(stream)
     reps = 5
     wall_b, dev_b = timed_loop(ta, lambda: b.launch(stream), reps, stream)
+    got = packed.download()
+    exact, off = True, 0
+    for sb in reqs_geo:
+        exp = orc.sb_pack(orc.sb_desc(*sb.as_tuple()), host, 0, 1)
+        exact &= bool((got[off:off + exp.size] == exp).all())
+        off += exp.size
+    del got
 
     def per_call():
         o = 0
@@ -683,7 +712,8 @@ def bench_random_requests(ta, stream, nreq=4096, seed=4):
     res = {"requests": nreq, "packed_bytes": total,
            "batched_packs_per_s": round(nreq * reps / wall_b, 1),
            "per_call_sync_packs_per_s": round(nreq / wall_pc, 1),
-           "batch_kernel_hbm_gbs": round(2 * total / (dev_b / reps) / 1e9, 1)}
+           "batch_kernel_hbm_gbs": round(2 * total / (dev_b / reps) / 1e9, 1),
+           "bit_exact": exact}
     res["speedup"] = round(res["batched_packs_per_s"] / res["per_call_sync_packs_per_s"], 1)
     log(f"random requests: {res}")
     b.destroy()
@@ -887,6 +917,191 @@ def bench_neighbor(ta, m, tc, reps=20):
     return res
 
 
+SWEEP_APP = os.path.join(ROOT, "tests", "c", "mpi_pack_sweep")
+
+
+def run_sweep_app(args, preload, extra_env=None, timeout=600):
+    """The native benchmark program (tests/c/mpi_pack_sweep.cpp) as a child
+    process, with libtempi_amd.so preloaded in front of the MPI library or
+    not; its JSON lines."""
+    import tempi_amd as ta
+
+    env = dict(os.environ)
+    if preload:
+        env["LD_PRELOAD"] = ta.LIB_PATH + (":" + env["LD_PRELOAD"] if env.get("LD_PRELOAD")
+                                           else "")
+    env.update(extra_env or {})
+    out = subprocess.run([SWEEP_APP, *args], env=env, capture_output=True, text=True,
+                         timeout=timeout)
+    if out.returncode != 0:
+        raise RuntimeError(f"{SWEEP_APP} {' '.join(args)}: exit {out.returncode}: "
+                           f"{out.stderr[-400:]}")
+    return [json.loads(line) for line in out.stdout.splitlines() if line.startswith("{")]
+
+
+def bench_reference_sweep():
+    """The reference's MPI_Pack benchmark (bin/bench_mpi_pack.cpp:127-143):
+    per-call MPI_Pack and MPI_Unpack wall time (trimean) of 2D byte types --
+    packed 1 KiB, 1 MiB, 4 MiB at incount 1 and 2, blocks of 1-512 B at a
+    512-B stride, as vector, hvector and subarray -- with device buffers and
+    with the packed buffer in mapped host memory through this library, and
+    with host buffers through the MPI library alone.  Throughput as the
+    reference reports it: packed bytes / pack time."""
+    import numpy as np
+
+    if not os.path.exists(SWEEP_APP):
+        return {"error": f"{SWEEP_APP} not built"}
+    log("reference MPI_Pack sweep: device, mapped host, MPI library on host buffers")
+    runs = {"device": run_sweep_app(["pack", "device", "100"], True),
+            "mapped": run_sweep_app(["pack", "mapped", "100"], True),
+            "mpi_library_host": run_sweep_app(["pack", "host", "5"], False)}
+    rows = {}
+    for mode, lines in runs.items():
+        for r in lines:
+            key = (r["count"], r["target"], r["block"])
+            row = rows.setdefault(key, {"count": r["count"], "target": r["target"],
+                                        "block": r["block"], "bytes": r["bytes"]})
+            short = {"2d_byte_vector": "vector", "2d_byte_hvector": "hvector",
+                     "2d_byte_subarray": "subarray"}[r["layout"]]
+            row[f"{mode}_{short}_pack_us"] = round(r["pack_s"] * 1e6, 2)
+            row[f"{mode}_{short}_unpack_us"] = round(r["unpack_s"] * 1e6, 2)
+            row[f"{mode}_{short}_pack_mibs"] = round(r["bytes"] / r["pack_s"] / 2 ** 20, 1)
+    table = sorted(rows.values(), key=lambda r: (r["target"], r["count"], r["block"]))
+    summary = {}
+    for target in (1024, 1 << 20, 4 << 20):
+        sel = [r for r in table if r["target"] == target]
+        sp = [r[f"device_{k}_pack_mibs"] / r[f"mpi_library_host_{k}_pack_mibs"]
+              for r in sel for k in ("vector", "hvector", "subarray")]
+        dev = [r["bytes"] / (r[f"device_{k}_pack_us"] * 1e-6) / 1e9
+               for r in sel for k in ("vector", "hvector", "subarray")]
+        summary[f"packed_{target}B"] = {
+            "device_pack_gbs_median": round(float(np.median(dev)), 2),
+            "speedup_vs_mpi_library_geomean": round(float(np.exp(np.mean(np.log(sp)))), 1),
+            "speedup_vs_mpi_library_min": round(float(np.min(sp)), 2),
+            "speedup_vs_mpi_library_max": round(float(np.max(sp)), 1)}
+    res = {"summary": summary, "rows": table,
+           "note": "wall time of one synchronous call from a native MPI program; device and "
+                   "mapped: this library preloaded; mpi_library_host: the MPI library alone "
+                   "on host buffers (5 iterations, as the reference runs its library-only "
+                   "case)"}
+    log(f"reference MPI_Pack sweep: {summary}")
+    return res
+
+
+def bench_type_commit():
+    """MPI_Type_commit wall time (trimean of 200) of the reference's commit
+    benchmark types (bin/bench_type_commit.cpp:28-53): through this library
+    with the learned selector evaluated at commit (the default), through it
+    with the closed-form rule (canonicalization only), and the MPI library
+    alone."""
+    import numpy as np
+
+    if not os.path.exists(SWEEP_APP):
+        return {"error": f"{SWEEP_APP} not built"}
+    log("type commit: model, rule, MPI library alone")
+    runs = {"with_model": run_sweep_app(["commit", "200"], True),
+            "canonicalize_only": run_sweep_app(["commit", "200"], True,
+                                               {"TEMPI_SELECT": "heuristic"}),
+            "mpi_library": run_sweep_app(["commit", "200"], False)}
+    res = {}
+    for k, lines in runs.items():
+        us = [r["commit_s"] * 1e6 for r in lines]
+        res[k] = {"cases": len(us), "median_us": round(float(np.median(us)), 2),
+                  "max_us": round(float(np.max(us)), 2)}
+        for layout in sorted({r["layout"] for r in lines}):
+            res[k][f"{layout}_median_us"] = round(float(np.median(
+                [r["commit_s"] * 1e6 for r in lines if r["layout"] == layout])), 2)
+    log(f"type commit: { {k: v['median_us'] for k, v in res.items()} }")
+    return res
+
+
+def bench_first_call(ta, m, tc):
+    """Wall time of the first synchronous MPI_Pack / MPI_Unpack of a
+    256 MiB shape (64-B rows, 2 x 64 + 64 B apart) against later calls:
+    under the default policy the first call of a copy of 64 MiB or more
+    times the model's candidates once, cold."""
+    import numpy as np
+
+    rows, bl = 1 << 22, 64
+    spec = ("vector", rows, bl, 2 * bl + 64, tc.BYTE)
+    h = m.type_from_spec(spec)
+    sb, size = m.strided_block(h)
+    src = ta.DeviceBuffer(sb.extent + 64)
+    dst = ta.DeviceBuffer(size)
+    res = {"packed_bytes": size}
+    for name, fn in (("pack", lambda: m.pack(src.addr(), 1, h, dst.addr(), size)),
+                     ("unpack", lambda: m.unpack(dst.addr(), size, src.addr(), 1, h))):
+        c0 = ta.counters()
+        t0 = time.perf_counter()
+        err, _ = fn()
+        first = time.perf_counter() - t0
+        assert err == 0
+        later = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            fn()
+            later.append(time.perf_counter() - t0)
+        c1 = ta.counters()
+        res[name] = {"first_call_ms": round(first * 1e3, 3),
+                     "later_call_ms_median": round(float(np.median(later)) * 1e3, 3),
+                     "trial_launches": c1["trial_launches"] - c0["trial_launches"]}
+    m.free(h)
+    src.free()
+    dst.free()
+    log(f"first call of a 256 MiB shape: {res}")
+    return res
+
+
+def bench_selector_regret(ta, n=512, rounds=3):
+    """The learned selector against the exhaustive autotuner on `n` held-out
+    copies (the selector's held-out generator, seed 900000: 2D/3D vectors,
+    halo pieces, grid faces and random blocks, a tenth of 64-256 MiB timed
+    cold).  Every variant of every copy is timed twice, independently (each
+    time the median of `rounds` timings): the autotuner picks from the first
+    timing, and every choice -- model, autotuner, closed-form rule -- is
+    judged on the second.  Regret = t(choice) / t(fastest) - 1."""
+    import numpy as np
+
+    from tempi_amd import selector_train as st
+
+    if not ta.lib().tempi_mlp_loaded():
+        return {"error": "no selector model loaded"}
+    cases = st.generate(n, 900000, 32 << 20)
+    src = ta.DeviceBuffer(st.SRC_BYTES)
+    src.fill(1)
+    dst = ta.DeviceBuffer(st.DST_BYTES + 4096)
+    dst.fill(2)
+    log(f"selector regret: timing {n} held-out copies x {ta.num_variants()} variants, twice")
+    _, T1, H, B = st.time_cases(cases, src, dst, log_every=0, rounds=rounds)
+    _, T2, _, _ = st.time_cases(cases, src, dst, log_every=0, rounds=rounds)
+    model = np.array([ta.select_variant(sb, count, src.addr(base), dst.addr(ppos), d,
+                                        ta.SELECT_MODEL)
+                      for sb, base, d, ppos, count in cases], np.int32)
+    auto = T1.argmin(1)
+    idx = np.arange(len(cases))
+
+    def stats(mask):
+        best = T2[mask].min(1)
+        out = {"copies": int(mask.sum())}
+        for name, pick in (("model", model), ("autotuner", auto), ("rule", H)):
+            t = T2[idx[mask], pick[mask]]
+            r = t / best - 1
+            out[name] = {"mean": round(float(r.mean()), 4),
+                         "median": round(float(np.median(r)), 4),
+                         "p90": round(float(np.quantile(r, 0.9)), 4),
+                         "within_5pct": round(float((r <= 0.05).mean()), 4),
+                         "time_weighted": round(float(t.sum() / best.sum() - 1), 4),
+                         "packed_gbs": round(float(B[mask].sum() / (t.sum() * 1e-6) / 1e9), 1)}
+        return out
+
+    res = {"timings_per_label": rounds, "all": stats(idx >= 0),
+           "ge_256KiB": stats(B >= (256 << 10)), "ge_64MiB_cold": stats(B >= st.BIG_MIN)}
+    src.free()
+    dst.free()
+    log(f"selector regret: { {k: v for k, v in res['ge_256KiB'].items() if k != 'copies'} }")
+    return res
+
+
 def bench_system_tables(ta):
     """Measure the method model's tables (kernel launch, d2h/h2d curves, the
     four 9x9 pack/unpack tables) and report a few of their points: 4 MiB in
@@ -919,7 +1134,9 @@ def bench_system_tables(ta):
 
 def cpu_baselines(args, m, orc, faces, host, np):
     """The single-core legs, pinned to one core of this process's CPU set
-    (restored afterwards)."""
+    (restored afterwards): the MPI library's own MPI_Pack / MPI_Unpack on
+    host buffers (the headline CPU figure), and the oracle's gather /
+    scatter."""
     allowed = os.sched_getaffinity(0)
     core = min(allowed)
     os.sched_setaffinity(0, {core})
@@ -930,28 +1147,13 @@ def cpu_baselines(args, m, orc, faces, host, np):
 
 
 def _cpu_baselines(args, m, orc, faces, host, np, core):
-    log(f"cpu baseline: oracle gather/scatter, {args.cpu_seconds:.0f} s sample, 1 thread "
+    log(f"cpu baseline: MPI library MPI_Pack + MPI_Unpack, {args.cpu_seconds:.0f} s sample, "
         f"pinned to core {core}")
-    descs = [orc.sb_desc(*f["sb"].as_tuple()) for f in faces]
     scratch = host.copy()
-    moved, reps = 0, 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < args.cpu_seconds:
-        for d, f in zip(descs, faces):
-            p = orc.sb_pack(d, host, 0, 1)
-            orc.sb_unpack(d, p, scratch, 0, 1)
-            moved += 2 * f["size"]
-        reps += 1
-    el = time.perf_counter() - t0
-    cpu = {"value": round(moved / el / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-           "cpu": host_cpu_model(),
-           "sample": f"oracle sb_pack + sb_unpack of the 3 faces of the 518^3 float32 grid, "
-                     f"{reps} steps in {el:.1f} s on 1 host thread pinned to core {core}"}
-    log(f"cpu baseline: MPI library MPI_Pack + MPI_Unpack, {args.cpu_seconds / 2:.0f} s sample")
     out = np.zeros(faces[0]["size"], np.uint8)
     moved, reps = 0, 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < args.cpu_seconds / 2:
+    while time.perf_counter() - t0 < args.cpu_seconds:
         for f in faces:
             m.pack(host.ctypes.data, 1, f["handle"], out.ctypes.data, f["size"], library=True)
             m.unpack(out.ctypes.data, f["size"], scratch.ctypes.data, 1, f["handle"],
@@ -960,9 +1162,40 @@ def _cpu_baselines(args, m, orc, faces, host, np, core):
         reps += 1
     el = time.perf_counter() - t0
     lib = {"value": round(moved / el / 1e9, 3), "unit": "GB/s", "cores": 1,
-           "library": "MPICH 3.3.2 MPI_Pack / MPI_Unpack on host buffers",
-           "sample": f"{reps} steps of the 3 faces in {el:.1f} s, pinned to core {core}"}
-    return cpu, lib
+           "kind": "reference", "cpu": host_cpu_model(),
+           "implementation": f"{mpi_library_name(m)} MPI_Pack / MPI_Unpack on host buffers, "
+                             "the reference's path for host data (src/pack.cpp:51-54)",
+           "sample": f"pack + unpack of the 3 faces of the 518^3 float32 grid, {reps} steps "
+                     f"in {el:.1f} s on 1 host thread pinned to core {core}"}
+    log(f"cpu baseline: oracle gather/scatter, {args.cpu_seconds / 2:.0f} s sample")
+    descs = [orc.sb_desc(*f["sb"].as_tuple()) for f in faces]
+    moved, reps = 0, 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds / 2:
+        for d, f in zip(descs, faces):
+            p = orc.sb_pack(d, host, 0, 1)
+            orc.sb_unpack(d, p, scratch, 0, 1)
+            moved += 2 * f["size"]
+        reps += 1
+    el = time.perf_counter() - t0
+    port = {"value": round(moved / el / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "implementation": "the oracle's C gather / scatter (oracle/oracle.c sb_pack, "
+                              "sb_unpack)",
+            "sample": f"the 3 faces, {reps} steps in {el:.1f} s on 1 host thread pinned to "
+                      f"core {core}"}
+    return lib, port
+
+
+def mpi_library_name(m):
+    import ctypes
+
+    buf = ctypes.create_string_buffer(8192)
+    n = ctypes.c_int()
+    try:
+        m.libmpi.MPI_Get_library_version(buf, ctypes.byref(n))
+        return buf.value.decode(errors="replace").splitlines()[0].strip()
+    except Exception:
+        return "the MPI library"
 
 
 def host_cpu_model():
@@ -977,36 +1210,73 @@ def host_cpu_model():
 
 
 def cpu_worker(spec):
-    """One process of the all-cores CPU baseline: builds its own copy of the
-    grid, reports ready, waits for the go line on stdin, then runs the
-    oracle's gather + scatter of the faces for the given time and prints the
-    bytes it moved.  Uses no GPU."""
-    from oracle import oracle as orc
+    """One process of the all-cores CPU baseline: the MPI library alone (no
+    libtempi_amd.so, no GPU), a singleton MPI_Init, the three face subarrays
+    committed, its own copy of the grid; reports ready, waits for the go line
+    on stdin, then runs MPI_Pack + MPI_Unpack of the faces for the given
+    time and prints the bytes it moved."""
+    import ctypes
+
+    import numpy as np
 
     job = json.loads(spec)
-    # the bytes' values do not change a gather's cost: counter-based fill
-    host = orc.splitmix_bytes(job["grid_bytes"], 2)
-    descs = [orc.sb_desc(*sb) for sb in job["faces"]]
+    L = ctypes.CDLL(job["libmpi"], mode=ctypes.RTLD_GLOBAL)
+    H = ctypes.c_int32 if job["handle_size"] == 4 else ctypes.c_int64
+    I3 = ctypes.c_int * 3
+    if L.MPI_Init(None, None) != 0:
+        raise RuntimeError("MPI_Init failed")
+    L.MPI_Type_create_subarray.argtypes = [ctypes.c_int, I3, I3, I3, ctypes.c_int, H,
+                                           ctypes.POINTER(H)]
+    L.MPI_Type_commit.argtypes = [ctypes.POINTER(H)]
+    L.MPI_Pack.argtypes = [ctypes.c_void_p, ctypes.c_int, H, ctypes.c_void_p, ctypes.c_int,
+                           ctypes.POINTER(ctypes.c_int), H]
+    L.MPI_Unpack.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                             ctypes.c_void_p, ctypes.c_int, H, H]
+    types = []
+    for sizes, subsizes, starts in job["faces"]:
+        t = H()
+        L.MPI_Type_create_subarray(3, I3(*sizes), I3(*subsizes), I3(*starts),
+                                   job["order_c"], H(job["mpi_float"]), ctypes.byref(t))
+        L.MPI_Type_commit(ctypes.byref(t))
+        types.append(t)
+    # the bytes' values do not change a gather's cost
+    host = np.ones(job["grid_bytes"], np.uint8)
+    scratch = np.ones(job["grid_bytes"], np.uint8)
+    out = np.zeros(max(job["sizes"]), np.uint8)
+    comm = H(job["comm_world"])
     print("ready", flush=True)
     sys.stdin.readline()
     moved = 0
+    pos = ctypes.c_int()
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < job["seconds"]:
-        for d, n in zip(descs, job["sizes"]):
-            orc.sb_unpack(d, orc.sb_pack(d, host, 0, 1), host, 0, 1)
+        for t, n in zip(types, job["sizes"]):
+            pos.value = 0
+            L.MPI_Pack(host.ctypes.data, 1, t, out.ctypes.data, n, ctypes.byref(pos), comm)
+            pos.value = 0
+            L.MPI_Unpack(out.ctypes.data, n, ctypes.byref(pos), scratch.ctypes.data, 1, t,
+                         comm)
             moved += 2 * n
-    print(json.dumps({"moved": moved, "seconds": time.perf_counter() - t0}), flush=True)
+    el = time.perf_counter() - t0
+    L.MPI_Finalize()
+    print(json.dumps({"moved": moved, "seconds": el}), flush=True)
 
 
-def cpu_all_cores(faces, grid_bytes, seconds):
-    """The same oracle gather/scatter in one single-threaded process per core
-    of this host's CPU share (at most 16), all timed over the same window;
-    the sum of their rates."""
+def cpu_all_cores(m, faces, grid_bytes, seconds):
+    """MPI_Pack + MPI_Unpack of the faces by the MPI library in one
+    single-rank process per core of this host's CPU share (at most 16), all
+    timed over the same window; the sum of their rates."""
     nproc = min(16, len(os.sched_getaffinity(0)))
-    log(f"cpu baseline: oracle gather/scatter in {nproc} processes, {seconds:.0f} s sample")
-    job = json.dumps({"grid_bytes": int(grid_bytes), "seconds": seconds,
-                      "faces": [list(f["sb"].as_tuple()) for f in faces],
-                      "sizes": [f["size"] for f in faces]})
+    log(f"cpu baseline: the MPI library in {nproc} processes, {seconds:.0f} s sample")
+    n, g = 512, 3
+    full = n + 2 * g
+    specs = {"x": [n, n, g], "y": [n, g, n], "z": [g, n, n]}
+    job = json.dumps({
+        "libmpi": m.libmpi._name, "handle_size": ta_handle_size(m),
+        "mpi_float": int(m.const("MPI_FLOAT")), "order_c": int(m.ORDER_C),
+        "comm_world": int(m.COMM_WORLD), "grid_bytes": int(grid_bytes), "seconds": seconds,
+        "faces": [[[full] * 3, specs[f["face"]], [g] * 3] for f in faces],
+        "sizes": [f["size"] for f in faces]})
     procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", job],
                               stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
              for _ in range(nproc)]
@@ -1023,16 +1293,26 @@ def cpu_all_cores(faces, grid_bytes, seconds):
             rates.append(r["moved"] / r["seconds"])
         for p in procs:
             p.wait(timeout=60)
+    except Exception as e:  # reported, not fatal: the GPU figures stand alone
+        log(f"cpu all-cores baseline failed: {e}")
+        return {"error": str(e)}
     finally:
         for p in procs:
             if p.poll() is None:
                 p.kill()
                 p.wait()
     return {"value": round(sum(rates) / 1e9, 3), "unit": "GB/s", "cores": nproc,
-            "kind": "port", "cpu": host_cpu_model(),
-            "sample": f"oracle sb_pack + sb_unpack of the 3 faces in {nproc} concurrent "
-                      f"single-threaded processes, each on its own copy of the grid, "
+            "kind": "reference", "cpu": host_cpu_model(),
+            "implementation": f"{mpi_library_name(m)} MPI_Pack / MPI_Unpack on host buffers",
+            "sample": f"pack + unpack of the 3 faces in {nproc} concurrent single-rank "
+                      f"processes (MPI library only), each on its own copy of the grid, "
                       f"{seconds:.0f} s window"}
+
+
+def ta_handle_size(m):
+    import ctypes
+
+    return ctypes.sizeof(m.H)
 
 
 if __name__ == "__main__":

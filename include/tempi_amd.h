@@ -215,6 +215,43 @@ int tempi_autotune(const tempi_strided_block *sb, int64_t incount,
                    const void *strided, void *packed, int direction,
                    int *best_variant, int *best_grid_cap,
                    float *per_variant_us);
+/* Buffers of tempi_time_variant and tempi_autotune: the copy runs as a pack
+   or unpack on them, so the strided buffer must hold every byte of the
+   type map from `strided` (start + extent-walk of `incount` objects, as
+   for tempi_pack_sb) and the packed buffer incount * packed-size bytes from
+   `packed`.  Both are written: the packed side by a pack, the strided
+   side's type-map bytes (only those) by an unpack.  The accesses of a copy
+   can be checked on the host with tempi_copy_emulate. */
+
+/* Host-side emulation of a copy: no counterpart in the reference.  Runs the
+   launches, grid, 32-bit grid-stride index sequence and per-unit address
+   arithmetic the library uses for (variant, grid cap) -- with the kernels'
+   own address code -- and records the bytes each launch would load and
+   store, without touching memory (the addresses are numbers only).
+   `max_blocks` is a workgroup cap (0: the default, 16 per CU) and
+   `cu_count` the CU count it assumes.  Copies of more than `unit_limit`
+   work units report only the launch plan (emulated = 0). */
+typedef struct tempi_copy_emulation {
+  int64_t launches;   /* kernel launches of the copy                       */
+  int64_t max_units;  /* most work units in one launch (stays < 2^31)      */
+  int32_t emulated;   /* 1: every unit emulated, fields below filled       */
+  int32_t wrapped;    /* 1: a 32-bit unit index would wrap                 */
+  int64_t read_lo, read_hi;   /* bytes loaded: [lo, hi) from the read side's
+                                 origin (strided origin for a pack, packed
+                                 pointer for an unpack)                     */
+  int64_t write_lo, write_hi; /* bytes stored, from the written side's origin */
+  int64_t bytes_read, bytes_written; /* totals over all loads and stores   */
+  int32_t exact;      /* 1: every byte the copy must write was stored once
+                         and no other byte; 0: not so; -1: not checked (the
+                         written span exceeds 256 MiB)                      */
+  int32_t reserved;
+} tempi_copy_emulation;
+int tempi_copy_emulate(const tempi_strided_block *sb, int64_t incount,
+                       uint64_t strided_addr, uint64_t packed_addr,
+                       int direction, int variant, int max_blocks,
+                       int cu_count, int64_t unit_limit,
+                       tempi_copy_emulation *out);
+
 /* Read `bytes` (<= 0: 1 GiB) of a library-owned buffer so that L2 and the
    Infinity Cache hold only clean, unrelated lines; synchronous.  For
    cold-cache timing. */
@@ -248,6 +285,16 @@ int tempi_batch_create(const tempi_batch_request *reqs, int64_t n, int method,
                        tempi_batch **batch);
 int tempi_batch_launch(tempi_batch *batch, void *stream);
 int tempi_batch_graph_launch(tempi_batch *batch, void *stream);
+/* One executable graph of several batches' launches, each after the one
+   before (for example the pack batch and the unpack batch of a step): one
+   tempi_graph_launch replays them all.  The graph refers to the batches'
+   device tables: the batches must outlive it.  No counterpart in the
+   reference. */
+typedef struct tempi_graph tempi_graph;
+int tempi_graph_create(tempi_batch *const *batches, int64_t n,
+                       tempi_graph **graph);
+int tempi_graph_launch(tempi_graph *graph, void *stream);
+int tempi_graph_destroy(tempi_graph *graph);
 int tempi_batch_info(const tempi_batch *batch, int64_t *requests,
                      int64_t *tiles, int64_t *packed_bytes);
 int tempi_batch_variant(const tempi_batch *batch, int64_t request,

@@ -121,6 +121,28 @@ class Counters(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class CopyEmulation(ctypes.Structure):
+    """tempi_copy_emulation (include/tempi_amd.h)."""
+
+    _fields_ = [
+        ("launches", ctypes.c_int64),
+        ("max_units", ctypes.c_int64),
+        ("emulated", ctypes.c_int32),
+        ("wrapped", ctypes.c_int32),
+        ("read_lo", ctypes.c_int64),
+        ("read_hi", ctypes.c_int64),
+        ("write_lo", ctypes.c_int64),
+        ("write_hi", ctypes.c_int64),
+        ("bytes_read", ctypes.c_int64),
+        ("bytes_written", ctypes.c_int64),
+        ("exact", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
 class BatchRequest(ctypes.Structure):
     """tempi_batch_request (include/tempi_amd.h)."""
 
@@ -183,6 +205,8 @@ def _prototypes(L):
                                     ctypes.POINTER(ctypes.c_float)]),
         "tempi_autotune": (_I, [sb_p, _I64, _P, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                 ctypes.POINTER(ctypes.c_float)]),
+        "tempi_copy_emulate": (_I, [sb_p, _I64, ctypes.c_uint64, ctypes.c_uint64, _I, _I, _I, _I,
+                                    _I64, ctypes.POINTER(CopyEmulation)]),
         "tempi_flush_caches": (_I, [_I64]),
         "tempi_launch_latency_us": (_I, [_I, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(ctypes.c_float)]),
@@ -192,6 +216,9 @@ def _prototypes(L):
         "tempi_batch_info": (_I, [_P, i64_p, i64_p, i64_p]),
         "tempi_batch_variant": (_I, [_P, _I64, ctypes.POINTER(_I)]),
         "tempi_batch_destroy": (_I, [_P]),
+        "tempi_graph_create": (_I, [ctypes.POINTER(_P), _I64, ctypes.POINTER(_P)]),
+        "tempi_graph_launch": (_I, [_P, _P]),
+        "tempi_graph_destroy": (_I, [_P]),
         "tempi_mlp_num_features": (_I, []),
         "tempi_mlp_num_outputs": (_I, []),
         "tempi_mlp_num_params": (_I, []),
@@ -409,6 +436,18 @@ def autotune(sb: StridedBlock, incount: int, strided: int, packed: int, directio
     return v.value, c.value, list(per)
 
 
+def copy_emulate(sb: StridedBlock, incount: int, strided_addr: int, packed_addr: int,
+                 direction: int, variant: int, max_blocks: int = 0, cu_count: int = 256,
+                 unit_limit: int = 1 << 22) -> dict:
+    """Host-side emulation of the copy's launches and memory accesses (no GPU,
+    nothing dereferenced); see tempi_copy_emulate."""
+    out = CopyEmulation()
+    check(lib().tempi_copy_emulate(ctypes.byref(sb), incount, strided_addr, packed_addr,
+                                   direction, variant, max_blocks, cu_count, unit_limit,
+                                   ctypes.byref(out)), "copy_emulate")
+    return out.as_dict()
+
+
 def flush_caches(nbytes: int = 0):
     """Evict L2 / Infinity Cache with a read-only sweep (for cold timings)."""
     check(lib().tempi_flush_caches(nbytes), "flush_caches")
@@ -420,6 +459,22 @@ def launch_latency_us(reps: int = 1000):
     return float(h.value), float(d.value)
 
 
+def batch_requests(requests):
+    """A ctypes array of tempi_batch_request from (StridedBlock, strided_addr,
+    packed_addr, count, direction[, variant]) tuples."""
+    reqs = list(requests)
+    arr = (BatchRequest * max(len(reqs), 1))()
+    for i, r in enumerate(reqs):
+        sb, strided, packed, count, direction = r[:5]
+        arr[i].sb = sb
+        arr[i].strided = strided
+        arr[i].packed = packed
+        arr[i].count = count
+        arr[i].direction = direction
+        arr[i].variant = r[5] if len(r) > 5 else -1
+    return arr, len(reqs)
+
+
 class Batch:
     """A fixed set of pack/unpack requests run by one persistent kernel.
 
@@ -427,20 +482,11 @@ class Batch:
     direction[, variant])."""
 
     def __init__(self, requests, method: int = SELECT_DEFAULT):
-        reqs = list(requests)
-        arr = (BatchRequest * max(len(reqs), 1))()
-        for i, r in enumerate(reqs):
-            sb, strided, packed, count, direction = r[:5]
-            arr[i].sb = sb
-            arr[i].strided = strided
-            arr[i].packed = packed
-            arr[i].count = count
-            arr[i].direction = direction
-            arr[i].variant = r[5] if len(r) > 5 else -1
+        arr, n = batch_requests(requests)
         h = ctypes.c_void_p()
-        check(lib().tempi_batch_create(arr, len(reqs), method, ctypes.byref(h)), "batch_create")
+        check(lib().tempi_batch_create(arr, n, method, ctypes.byref(h)), "batch_create")
         self.h = h
-        self.n = len(reqs)
+        self.n = n
 
     def launch(self, stream_=None):
         check(lib().tempi_batch_launch(self.h, stream_), "batch_launch")
@@ -461,6 +507,34 @@ class Batch:
     def destroy(self):
         if self.h:
             lib().tempi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Graph:
+    """One executable graph replaying the launches of several batches in
+    order (the batches must outlive it)."""
+
+    def __init__(self, batches):
+        self.batches = list(batches)  # kept alive with the graph
+        arr = (ctypes.c_void_p * max(len(self.batches), 1))(
+            *[b.h.value for b in self.batches])
+        h = ctypes.c_void_p()
+        check(lib().tempi_graph_create(arr, len(self.batches), ctypes.byref(h)),
+              "graph_create")
+        self.h = h
+
+    def launch(self, stream_=None):
+        check(lib().tempi_graph_launch(self.h, stream_), "graph_launch")
+
+    def destroy(self):
+        if self.h:
+            lib().tempi_graph_destroy(self.h)
             self.h = None
 
     def __del__(self):

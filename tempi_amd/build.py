@@ -36,6 +36,8 @@ SOURCES = [
 ]
 
 ARCH = os.environ.get("TEMPI_ARCH", "gfx950")
+# ROCm libraries beside the HIP runtime (the ROCTx range API)
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
 
 
 def mpi_home() -> str:
@@ -105,7 +107,8 @@ def build(verbose: bool = False) -> str:
     cmd = [
         hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs,
         f"-L{os.path.join(mpi, 'lib')}", "-lmpi",
-        f"-Wl,-rpath,{':'.join(rpath)}", "-Wl,--enable-new-dtags",
+        f"-L{ROCM_LIB}", "-lrocprofiler-sdk-roctx",
+        f"-Wl,-rpath,{':'.join(rpath + [ROCM_LIB])}", "-Wl,--enable-new-dtags",
         "-ldl", "-o", tmp,
     ]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -119,22 +122,29 @@ def build(verbose: bool = False) -> str:
 
 TEST_APP_SRC = os.path.join(ROOT, "tests", "c", "pack_unpack_app.cpp")
 TEST_APP = os.path.join(ROOT, "tests", "c", "pack_unpack_app")
+SWEEP_APP_SRC = os.path.join(ROOT, "tests", "c", "mpi_pack_sweep.cpp")
+SWEEP_APP = os.path.join(ROOT, "tests", "c", "mpi_pack_sweep")
 
 
 def build_test_app(verbose: bool = False) -> str:
-    """The native MPI test program (tests/c/pack_unpack_app.cpp), linked
-    against the MPI library only: the GPU test runs it with libtempi_amd.so
-    preloaded, the drop-in way."""
+    """The native MPI programs (tests/c/pack_unpack_app.cpp, the parity
+    test; tests/c/mpi_pack_sweep.cpp, the reference's pack and commit
+    benchmarks), linked against the MPI library and the HIP runtime only:
+    they run with libtempi_amd.so preloaded, the drop-in way."""
     mpi = mpi_home()
     rpath = [d for d in (system_libdir(), os.path.join(mpi, "lib")) if d]
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", TEST_APP_SRC,
-           f"-I{os.path.join(mpi, 'include')}", f"-L{os.path.join(mpi, 'lib')}", "-lmpi",
-           f"-Wl,-rpath,{':'.join(rpath)}", "-Wl,--enable-new-dtags", "-o", TEST_APP]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"test app build failed: {' '.join(cmd)}\n{res.stderr}")
-    if verbose:
-        print(f"built {TEST_APP}")
+    for src, out in ((TEST_APP_SRC, TEST_APP), (SWEEP_APP_SRC, SWEEP_APP)):
+        # libmpi by path: a -L of the MPI directory would also resolve
+        # libstdc++ from there (conda ships an older one)
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", src,
+               f"-I{os.path.join(mpi, 'include')}",
+               f"-Wl,{os.path.join(mpi, 'lib', 'libmpi.so')}",
+               f"-Wl,-rpath,{':'.join(rpath)}", "-Wl,--enable-new-dtags", "-o", out]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"native program build failed: {' '.join(cmd)}\n{res.stderr}")
+        if verbose:
+            print(f"built {out}")
     return TEST_APP
 
 

@@ -63,6 +63,18 @@ FlushBuf &flush_buf() {
   return f;
 }
 
+// Release the sweep buffer (after the stream's work is done).
+void flush_release() {
+  FlushBuf &f = flush_buf();
+  std::lock_guard<std::mutex> lk(f.mu);
+  if (f.buf) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(f.buf);
+    f.buf = nullptr;
+    f.bytes = 0;
+  }
+}
+
 // Sweep `bytes` (<= 0: 1 GiB) of a library-owned buffer with k_touch on
 // `stream`; the buffer is allocated and filled on first use.
 int flush_caches_on(int64_t bytes, hipStream_t s) {
@@ -189,14 +201,18 @@ int autotune(const Geom &g, const char *strided, char *packed, int dir,
   return TEMPI_SUCCESS;
 }
 
-// Trial copies of the library's own choices (verification and the
-// autotune policy) write a scratch image of the destination, never the
-// caller's buffers: a batch resolves its variants when it is created, long
-// before it runs, and an unpack trial would otherwise overwrite the
-// caller's strided buffer with whatever the packed buffer held then.  The
-// image has the written side's length and its address modulo 256, so the
-// trials see the same alignment (and the same row, line and page pattern)
-// as the real copy; the read side is the caller's, untouched.
+// Where trial copies of the library's own choices (verification and the
+// autotune policy) write.  A copy that runs right after its choice (MPI_Pack,
+// MPI_Unpack, tempi_pack_sb, tempi_unpack_sb) is timed on the caller's own
+// buffers: each trial stores exactly the bytes the real copy stores next,
+// from the same source, so nothing extra is allocated.  A batch resolves
+// its variants when it is created, long before it runs -- an unpack trial
+// there would overwrite the caller's strided buffer with whatever the
+// packed buffer held then -- so its trials write a scratch image of the
+// destination from the staging pool instead, at most kMaxTrialScratch
+// bytes.  The image has the written side's length and its address modulo
+// 256, so the trials see the same alignment (and the same row, line and
+// page pattern) as the real copy; the read side is the caller's, untouched.
 TrialBuffers &TrialBuffers::operator=(TrialBuffers &&o) noexcept {
   if (this != &o) {
     this->~TrialBuffers();
@@ -212,7 +228,7 @@ TrialBuffers &TrialBuffers::operator=(TrialBuffers &&o) noexcept {
 TrialBuffers::~TrialBuffers() {
   if (mem) {
     (void)hipStreamSynchronize(stream);
-    (void)hipFree(mem);
+    pool_release(false, mem);
     mem = nullptr;
   }
 }
@@ -229,8 +245,9 @@ int64_t written_span(const Geom &g, int dir) {
   return hi - lo;
 }
 
-// largest scratch image a trial may allocate
-constexpr size_t kMaxTrialScratch = 4ull << 30;
+// largest scratch image a deferred trial may take from the pool (a
+// 256 MiB packed stream and its alignment slack)
+constexpr size_t kMaxTrialScratch = (256ull << 20) + 256;
 
 int trial_buffers(const Geom &g, const char *strided, char *packed, int dir,
                   hipStream_t stream, TrialBuffers *t) {
@@ -249,14 +266,13 @@ int trial_buffers(const Geom &g, const char *strided, char *packed, int dir,
   const size_t bytes = (size_t)(hi - lo) + 256;
   t->stream = stream;
   if (bytes > kMaxTrialScratch) {
-    // a sparse unpack can span far more than it writes; its trials are not
-    // worth that much memory, and the copy keeps the model's pick
+    // trials of a copy this large are not worth the memory; it keeps the
+    // model's pick
     return set_error(TEMPI_ERR_NOMEM,
                      "trial copies would need %zu B of scratch", bytes);
   }
-  if (hipSuccess != hipMalloc(&t->mem, bytes)) {
-    (void)hipGetLastError();
-    t->mem = nullptr;
+  t->mem = pool_acquire(false, bytes);
+  if (!t->mem) {
     return set_error(TEMPI_ERR_NOMEM, "no scratch for trial copies (%zu B)",
                      bytes);
   }
@@ -285,7 +301,7 @@ int trial_buffers(const Geom &g, const char *strided, char *packed, int dir,
 
 static int verify_candidates(const Geom &g, const char *strided, char *packed,
                              int dir, int rule, hipStream_t stream,
-                             Choice *out) {
+                             bool in_place, Choice *out) {
   int cand[4];
   int n = model_ranked(g, strided, packed, dir, 3, cand);
   if (n == 0) {
@@ -295,29 +311,38 @@ static int verify_candidates(const Geom &g, const char *strided, char *packed,
     cand[n++] = rule;
   }
   TrialBuffers t;
-  int err = trial_buffers(g, strided, packed, dir, stream, &t);
-  if (err) {
-    return err;
-  }
-  float best_us = 1e30f;
-  for (int i = 0; i < n; ++i) {
-    float us = 0;
-    err = time_config(g, t.strided, t.packed, dir, cand[i], 0, 1, stream, &us);
+  const char *ts = strided;
+  char *tp = packed;
+  if (!in_place) {
+    int err = trial_buffers(g, strided, packed, dir, stream, &t);
     if (err) {
       return err;
     }
-    if (us < best_us) {
+    ts = t.strided;
+    tp = t.packed;
+  }
+  float best_us = 1e30f;
+  int err = TEMPI_SUCCESS;
+  for (int i = 0; i < n && !err; ++i) {
+    float us = 0;
+    err = time_config(g, ts, tp, dir, cand[i], 0, 1, stream, &us);
+    if (!err && us < best_us) {
       best_us = us;
       out->variant = cand[i];
       out->max_blocks = 0;
     }
+  }
+  // the cold trials' cache sweep buffer is not kept for later calls
+  flush_release();
+  if (err) {
+    return err;
   }
   ++counters().verified_choices;
   return TEMPI_SUCCESS;
 }
 
 Choice choose(const Geom &g, const char *strided, char *packed, int dir,
-              int method, hipStream_t stream) {
+              int method, hipStream_t stream, bool in_place) {
   Choice c;
   c.variant = heuristic_variant(g, strided, packed, dir);
   if (method == TEMPI_SELECT_MODEL ||
@@ -332,8 +357,8 @@ Choice choose(const Geom &g, const char *strided, char *packed, int dir,
         }
       }
       Choice v;
-      if (stream &&
-          !verify_candidates(g, strided, packed, dir, c.variant, stream, &v)) {
+      if (stream && !verify_candidates(g, strided, packed, dir, c.variant,
+                                       stream, in_place, &v)) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_verify_cache[key] = v;
         return v;
@@ -358,8 +383,9 @@ Choice choose(const Geom &g, const char *strided, char *packed, int dir,
   }
   Choice best = c;
   TrialBuffers t;
-  if (trial_buffers(g, strided, packed, dir, stream, &t) ||
-      autotune(g, t.strided, t.packed, dir, stream, &best, nullptr) ||
+  if ((!in_place && trial_buffers(g, strided, packed, dir, stream, &t)) ||
+      autotune(g, in_place ? strided : t.strided,
+               in_place ? packed : t.packed, dir, stream, &best, nullptr) ||
       hipSuccess != hipStreamSynchronize(stream)) {
     TEMPI_LOG(LOG_WARN, "autotune failed (%s); using the closed-form rule",
               tempi_last_error());

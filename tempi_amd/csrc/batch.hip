@@ -177,45 +177,31 @@ __global__ void __launch_bounds__(kBlock)
 
 using namespace tempi;
 
-struct tempi_batch {
-  int64_t requests = 0;
-  int64_t bytes = 0;
-  uint32_t ntiles = 0;
+namespace tempi {
+namespace {
+
+// What one launch of a batch needs: the grid, the kernel, and where its
+// tables sit in device memory.
+struct BatchLaunch {
   int grid = 0;
-  int n = 0; // items with work
-  char *dev = nullptr; // items and tile tables in one allocation
-  BatchItem *d_items = nullptr;
-  Tile *d_tiles = nullptr;          // tile records, grouped by XCD
-  uint32_t *d_xcd_begin = nullptr;  // first tile of each XCD (kXcds + 1)
+  uint32_t ntiles = 0;
   int xcd_split = 0;
-  bool small = false; // launched as k_batch_small with `ix`
+  bool small = false; // k_batch_small with `ix`
   SmallIndex ix{};
-  hipStream_t capture_stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  std::vector<int> variants;
+  const BatchItem *items = nullptr;
+  const Tile *tiles = nullptr;        // tile records, grouped by XCD
+  const uint32_t *xcd_begin = nullptr; // first tile of each XCD (kXcds + 1)
 };
 
-static void launch_kernel(const tempi_batch *b, hipStream_t s) {
-  if (b->small) {
-    hipLaunchKernelGGL(k_batch_small, dim3(b->grid), dim3(kBlock), 0, s, b->ix,
-                       b->d_items);
+void launch_kernel(const BatchLaunch &l, hipStream_t s) {
+  if (l.small) {
+    hipLaunchKernelGGL(k_batch_small, dim3(l.grid), dim3(kBlock), 0, s, l.ix,
+                       l.items);
   } else {
-    hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kBlock), 0, s, b->d_items,
-                       b->d_tiles, b->d_xcd_begin, b->ntiles, b->xcd_split);
+    hipLaunchKernelGGL(k_batch, dim3(l.grid), dim3(kBlock), 0, s, l.items,
+                       l.tiles, l.xcd_begin, l.ntiles, l.xcd_split);
   }
 }
-
-static int batch_launch_on(tempi_batch *b, hipStream_t s) {
-  if (b->ntiles == 0) {
-    return TEMPI_SUCCESS;
-  }
-  launch_kernel(b, s);
-  TEMPI_HIP_CHECK(hipGetLastError());
-  return TEMPI_SUCCESS;
-}
-
-namespace {
 
 // One request as the batch resolves it.
 struct Req {
@@ -226,18 +212,19 @@ struct Req {
   int variant = 0;
 };
 
-// The device tables and launch shape of a batch of resolved requests;
-// replaces what `b` held (waiting for the library stream first).
-int build_batch(tempi_batch *b, const std::vector<Req> &rs) {
-  hipStream_t s = library_stream();
-  if (b->dev) {
-    (void)hipStreamSynchronize(s);
-    (void)hipFree(b->dev);
-    b->dev = nullptr;
-  }
-  b->small = false;
-  b->xcd_split = 0;
-  b->ix = SmallIndex{};
+// The host image of a batch's device tables (request parameters, tile
+// records, XCD range bounds; each section 256-B aligned) and its launch
+// shape, with the table pointers left as offsets into the image.
+struct BatchPlan {
+  std::vector<char> image;
+  size_t tl_off = 0, xb_off = 0;
+  BatchLaunch launch;
+  int n = 0;
+};
+
+int plan_batch(const std::vector<Req> &rs, int cu_count, BatchPlan *pl) {
+  BatchLaunch &L = pl->launch;
+  L = BatchLaunch{};
   std::vector<BatchItem> items;
   std::vector<uint32_t> item_tiles; // tiles of each item
   uint64_t tiles = 0;
@@ -261,15 +248,15 @@ int build_batch(tempi_batch *b, const std::vector<Req> &rs) {
     item_tiles.push_back((uint32_t)t);
     items.push_back(it);
   }
-  b->n = (int)items.size();
-  b->ntiles = (uint32_t)tiles;
+  pl->n = (int)items.size();
+  L.ntiles = (uint32_t)tiles;
   // persistent grid: up to 8 workgroups per CU, a multiple of the XCD count
   // when the tiles are split by XCD
-  const int64_t cap = (int64_t)device_cu_count() * 8;
+  const int64_t cap = (int64_t)cu_count * 8;
   if (!items.empty() && items.size() <= (size_t)kSmallItems) {
     // tile index in the kernel arguments; only the parameters go to the
     // device table
-    SmallIndex &ix = b->ix;
+    SmallIndex &ix = L.ix;
     ix.n = (uint32_t)items.size();
     uint32_t base = 0;
     for (size_t r = 0; r < items.size(); ++r) {
@@ -282,19 +269,19 @@ int build_batch(tempi_batch *b, const std::vector<Req> &rs) {
       ix.base[r] = base;
     }
     ix.end = base;
-    b->small = true;
-    b->grid = (int)std::min<int64_t>((int64_t)base, cap / kXcds * kXcds);
+    L.small = true;
+    L.grid = (int)std::min<int64_t>((int64_t)base, cap / kXcds * kXcds);
   } else if ((int64_t)tiles >= 8 * (int64_t)kXcds) {
-    b->xcd_split = 1;
-    b->grid = (int)(std::min<int64_t>((int64_t)tiles, cap) / kXcds * kXcds);
+    L.xcd_split = 1;
+    L.grid = (int)(std::min<int64_t>((int64_t)tiles, cap) / kXcds * kXcds);
   } else {
-    b->grid = (int)std::max<int64_t>(1, (int64_t)tiles);
+    L.grid = (int)std::max<int64_t>(1, (int64_t)tiles);
   }
   // tile records grouped by XCD: XCD x takes the x-th eighth of every
   // item's tiles (a single group when the grid is not split)
-  const int groups = b->small ? 0 : b->xcd_split ? (int)kXcds : 1;
+  const int groups = L.small ? 0 : L.xcd_split ? (int)kXcds : 1;
   std::vector<Tile> tile_list;
-  tile_list.reserve(b->small ? 0 : tiles);
+  tile_list.reserve(L.small ? 0 : tiles);
   std::vector<uint32_t> xcd_begin(kXcds + 1, (uint32_t)tiles);
   for (int x = 0; x < groups; ++x) {
     xcd_begin[x] = (uint32_t)tile_list.size();
@@ -307,47 +294,39 @@ int build_batch(tempi_batch *b, const std::vector<Req> &rs) {
     }
   }
   const size_t items_bytes = items.size() * sizeof(BatchItem);
-  const size_t tl_off = (items_bytes + 255) / 256 * 256;
-  const size_t xb_off =
-      tl_off + (tile_list.size() * sizeof(Tile) + 255) / 256 * 256;
-  const size_t total = xb_off + 256;
-  if (hipSuccess != hipMalloc(&b->dev, total)) {
-    (void)hipGetLastError();
-    b->dev = nullptr;
-    return set_error(TEMPI_ERR_NOMEM, "batch table allocation failed");
-  }
-  b->d_items = reinterpret_cast<BatchItem *>(b->dev);
-  b->d_tiles = reinterpret_cast<Tile *>(b->dev + tl_off);
-  b->d_xcd_begin = reinterpret_cast<uint32_t *>(b->dev + xb_off);
-  std::vector<char> host(total, 0);
+  pl->tl_off = (items_bytes + 255) / 256 * 256;
+  pl->xb_off =
+      pl->tl_off + (tile_list.size() * sizeof(Tile) + 255) / 256 * 256;
+  pl->image.assign(pl->xb_off + 256, 0);
   if (!items.empty()) {
-    memcpy(host.data(), items.data(), items_bytes);
-    if (!tile_list.empty()) {
-      memcpy(host.data() + tl_off, tile_list.data(),
-             tile_list.size() * sizeof(Tile));
-    }
+    memcpy(pl->image.data(), items.data(), items_bytes);
   }
-  memcpy(host.data() + xb_off, xcd_begin.data(), xcd_begin.size() * 4);
-  // one upload, outside any capture
-  if (hipSuccess != hipMemcpyAsync(b->dev, host.data(), total,
-                                   hipMemcpyHostToDevice, s) ||
-      hipSuccess != hipStreamSynchronize(s)) {
-    (void)hipGetLastError();
-    (void)hipFree(b->dev);
-    b->dev = nullptr;
-    return set_error(TEMPI_ERR_HIP, "batch table upload failed");
+  if (!tile_list.empty()) {
+    memcpy(pl->image.data() + pl->tl_off, tile_list.data(),
+           tile_list.size() * sizeof(Tile));
   }
+  memcpy(pl->image.data() + pl->xb_off, xcd_begin.data(),
+         xcd_begin.size() * 4);
   return TEMPI_SUCCESS;
 }
 
-} // namespace
+// the launch of a plan whose image was copied to device address `dev`
+BatchLaunch placed(const BatchPlan &pl, char *dev) {
+  BatchLaunch L = pl.launch;
+  L.items = reinterpret_cast<const BatchItem *>(dev);
+  L.tiles = reinterpret_cast<const Tile *>(dev + pl.tl_off);
+  L.xcd_begin = reinterpret_cast<const uint32_t *>(dev + pl.xb_off);
+  return L;
+}
 
-TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
-                                    int method, tempi_batch **out) {
-  if (!out || (n > 0 && !reqs) || n < 0) {
-    return set_error(TEMPI_ERR_ARG, "bad argument");
-  }
-  *out = nullptr;
+// Check and resolve the caller's requests: geometry, device addresses and,
+// for a request that leaves it to the library (variant -1), the variant
+// `pick` returns.  Requests that move nothing are left out; `where` gets
+// the caller's index of each resolved request.
+template <class Pick>
+int resolve_requests(const tempi_batch_request *reqs, int64_t n, Pick &&pick,
+                     std::vector<Req> *rs, std::vector<int64_t> *where,
+                     int64_t *bytes) {
   // every request is checked before the device is touched
   int err = TEMPI_SUCCESS;
   std::vector<Geom> geoms((size_t)n);
@@ -360,13 +339,7 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   if ((err = ensure_runtime())) {
     return err;
   }
-  hipStream_t s = library_stream();
-  std::unique_ptr<tempi_batch> owner(new tempi_batch());
-  tempi_batch *b = owner.get();
-  b->requests = n;
-  b->variants.assign((size_t)n, -1);
-  std::vector<Req> rs;
-  std::vector<int64_t> where; // caller's index of each resolved request
+  *bytes = 0;
   for (int64_t i = 0; i < n; ++i) {
     const tempi_batch_request &r = reqs[i];
     Req q;
@@ -390,15 +363,95 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
     q.dir = r.direction;
     q.variant = r.variant;
     if (q.variant < 0 || q.variant >= kNumVariants) {
-      // the per-copy policy at every size: on MI355X the learned
-      // selector's picks for requests under 1 MiB ran batches of grid
-      // faces 5-28 % faster than the closed-form rule's (faces of 128^3 -
-      // 256^3 float grids) and the 64-subdomain halo batch within 1-2 %
-      q.variant = choose(q.g, q.ds, q.dp, q.dir, method, s).variant;
+      q.variant = pick(q);
     }
-    b->bytes += q.g.L * q.g.rows;
-    rs.push_back(q);
-    where.push_back(i);
+    *bytes += q.g.L * q.g.rows;
+    rs->push_back(q);
+    where->push_back(i);
+  }
+  return TEMPI_SUCCESS;
+}
+
+} // namespace
+} // namespace tempi
+
+struct tempi_batch {
+  int64_t requests = 0;
+  int64_t bytes = 0;
+  int n = 0;           // items with work
+  char *dev = nullptr; // the tables, one allocation
+  BatchLaunch launch;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> variants;
+};
+
+static int batch_launch_on(tempi_batch *b, hipStream_t s) {
+  if (b->launch.ntiles == 0) {
+    return TEMPI_SUCCESS;
+  }
+  launch_kernel(b->launch, s);
+  TEMPI_HIP_CHECK(hipGetLastError());
+  return TEMPI_SUCCESS;
+}
+
+namespace {
+
+// The device tables and launch shape of a batch of resolved requests.
+int build_batch(tempi_batch *b, const std::vector<Req> &rs) {
+  hipStream_t s = library_stream();
+  BatchPlan pl;
+  int err = plan_batch(rs, device_cu_count(), &pl);
+  if (err) {
+    return err;
+  }
+  b->n = pl.n;
+  if (hipSuccess != hipMalloc(&b->dev, pl.image.size())) {
+    (void)hipGetLastError();
+    b->dev = nullptr;
+    return set_error(TEMPI_ERR_NOMEM, "batch table allocation failed");
+  }
+  b->launch = placed(pl, b->dev);
+  // one upload, outside any capture
+  if (hipSuccess != hipMemcpyAsync(b->dev, pl.image.data(), pl.image.size(),
+                                   hipMemcpyHostToDevice, s) ||
+      hipSuccess != hipStreamSynchronize(s)) {
+    (void)hipGetLastError();
+    (void)hipFree(b->dev);
+    b->dev = nullptr;
+    return set_error(TEMPI_ERR_HIP, "batch table upload failed");
+  }
+  return TEMPI_SUCCESS;
+}
+
+} // namespace
+
+TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
+                                    int method, tempi_batch **out) {
+  if (!out || (n > 0 && !reqs) || n < 0) {
+    return set_error(TEMPI_ERR_ARG, "bad argument");
+  }
+  *out = nullptr;
+  std::unique_ptr<tempi_batch> owner(new tempi_batch());
+  tempi_batch *b = owner.get();
+  b->requests = n;
+  b->variants.assign((size_t)n, -1);
+  std::vector<Req> rs;
+  std::vector<int64_t> where;
+  // the per-copy policy at every size: on MI355X the learned selector's
+  // picks for requests under 1 MiB ran batches of grid faces 5-28 % faster
+  // than the closed-form rule's (faces of 128^3 - 256^3 float grids) and the
+  // 64-subdomain halo batch within 1-2 %
+  int err = resolve_requests(
+      reqs, n,
+      [&](const Req &q) {
+        return choose(q.g, q.ds, q.dp, q.dir, method, library_stream(),
+                      /*in_place=*/false)
+            .variant;
+      },
+      &rs, &where, &b->bytes);
+  if (err) {
+    return err;
   }
   for (size_t k = 0; k < rs.size(); ++k) {
     b->variants[(size_t)where[k]] = rs[k].variant;
@@ -414,6 +467,7 @@ TEMPI_EXPORT int tempi_batch_launch(tempi_batch *b, void *stream) {
   if (!b) {
     return set_error(TEMPI_ERR_ARG, "null batch");
   }
+  TraceRange trace("tempi_batch_launch");
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : library_stream();
   int err = batch_launch_on(b, s);
   if (!err) {
@@ -422,25 +476,149 @@ TEMPI_EXPORT int tempi_batch_launch(tempi_batch *b, void *stream) {
   return err;
 }
 
+namespace {
+
+// A graph of the launches of `n` batches, each depending on the one before
+// (kernel nodes added directly: no stream capture).  The batches' device
+// tables must outlive the graph.
+int build_graph(tempi_batch *const *bs, int64_t n, hipGraph_t *graph,
+                hipGraphExec_t *exec, int64_t *nodes) {
+  *graph = nullptr;
+  *exec = nullptr;
+  *nodes = 0;
+  TEMPI_HIP_CHECK(hipGraphCreate(graph, 0));
+  hipGraphNode_t prev = nullptr;
+  int err = TEMPI_SUCCESS;
+  for (int64_t i = 0; i < n && !err; ++i) {
+    const BatchLaunch &l = bs[i]->launch;
+    if (l.ntiles == 0) {
+      continue;
+    }
+    hipKernelNodeParams kp{};
+    void *args_small[] = {const_cast<SmallIndex *>(&l.ix),
+                          const_cast<const BatchItem **>(&l.items)};
+    uint32_t ntiles = l.ntiles;
+    int xcd_split = l.xcd_split;
+    void *args_big[] = {const_cast<const BatchItem **>(&l.items),
+                        const_cast<const Tile **>(&l.tiles),
+                        const_cast<const uint32_t **>(&l.xcd_begin), &ntiles,
+                        &xcd_split};
+    kp.func = l.small ? reinterpret_cast<void *>(&k_batch_small)
+                      : reinterpret_cast<void *>(&k_batch);
+    kp.gridDim = dim3((unsigned)l.grid);
+    kp.blockDim = dim3(kBlock);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = l.small ? args_small : args_big;
+    kp.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    const hipError_t e = hipGraphAddKernelNode(&node, *graph, prev ? &prev : nullptr,
+                                               prev ? 1 : 0, &kp);
+    if (e != hipSuccess) {
+      err = set_error(TEMPI_ERR_HIP, "hipGraphAddKernelNode: %s",
+                      hipGetErrorString(e));
+      break;
+    }
+    prev = node;
+    ++*nodes;
+  }
+  if (!err) {
+    const hipError_t e = hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      err = set_error(TEMPI_ERR_HIP, "hipGraphInstantiate: %s",
+                      hipGetErrorString(e));
+    }
+  }
+  if (err) {
+    (void)hipGetLastError();
+    if (*exec) {
+      (void)hipGraphExecDestroy(*exec);
+    }
+    (void)hipGraphDestroy(*graph);
+    *graph = nullptr;
+    *exec = nullptr;
+  }
+  return err;
+}
+
+} // namespace
+
 TEMPI_EXPORT int tempi_batch_graph_launch(tempi_batch *b, void *stream) {
   if (!b) {
     return set_error(TEMPI_ERR_ARG, "null batch");
   }
-  if (b->ntiles == 0) {
+  if (b->launch.ntiles == 0) {
     return TEMPI_SUCCESS;
   }
+  TraceRange trace("tempi_batch_graph_launch");
   if (!b->exec) {
-    TEMPI_HIP_CHECK(
-        hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
-    TEMPI_HIP_CHECK(hipStreamBeginCapture(b->capture_stream,
-                                          hipStreamCaptureModeThreadLocal));
-    launch_kernel(b, b->capture_stream);
-    TEMPI_HIP_CHECK(hipStreamEndCapture(b->capture_stream, &b->graph));
-    TEMPI_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    int64_t nodes = 0;
+    int err = build_graph(&b, 1, &b->graph, &b->exec, &nodes);
+    if (err) {
+      return err;
+    }
   }
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : library_stream();
   TEMPI_HIP_CHECK(hipGraphLaunch(b->exec, s));
   ++counters().graph_launches;
+  return TEMPI_SUCCESS;
+}
+
+struct tempi_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t nodes = 0;
+};
+
+TEMPI_EXPORT int tempi_graph_create(tempi_batch *const *batches, int64_t n,
+                                    tempi_graph **out) {
+  if (!out || n < 0 || (n > 0 && !batches)) {
+    return set_error(TEMPI_ERR_ARG, "bad argument");
+  }
+  *out = nullptr;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!batches[i]) {
+      return set_error(TEMPI_ERR_ARG, "null batch %lld", (long long)i);
+    }
+  }
+  int err = ensure_runtime();
+  if (err) {
+    return err;
+  }
+  std::unique_ptr<tempi_graph> g(new tempi_graph());
+  if ((err = build_graph(batches, n, &g->graph, &g->exec, &g->nodes))) {
+    return err;
+  }
+  *out = g.release();
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_graph_launch(tempi_graph *g, void *stream) {
+  if (!g) {
+    return set_error(TEMPI_ERR_ARG, "null graph");
+  }
+  if (g->nodes == 0) {
+    return TEMPI_SUCCESS;
+  }
+  TraceRange trace("tempi_graph_launch");
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : library_stream();
+  TEMPI_HIP_CHECK(hipGraphLaunch(g->exec, s));
+  ++counters().graph_launches;
+  return TEMPI_SUCCESS;
+}
+
+TEMPI_EXPORT int tempi_graph_destroy(tempi_graph *g) {
+  if (!g) {
+    return TEMPI_SUCCESS;
+  }
+  if (g->exec) {
+    // the graph may have been launched on any stream
+    (void)hipDeviceSynchronize();
+    (void)hipGraphExecDestroy(g->exec);
+  }
+  if (g->graph) {
+    (void)hipGraphDestroy(g->graph);
+  }
+  delete g;
   return TEMPI_SUCCESS;
 }
 
@@ -453,7 +631,7 @@ TEMPI_EXPORT int tempi_batch_info(const tempi_batch *b, int64_t *requests,
     *requests = b->requests;
   }
   if (tiles) {
-    *tiles = b->ntiles;
+    *tiles = b->launch.ntiles;
   }
   if (packed_bytes) {
     *packed_bytes = b->bytes;
@@ -474,18 +652,17 @@ TEMPI_EXPORT int tempi_batch_destroy(tempi_batch *b) {
   if (!b) {
     return TEMPI_SUCCESS;
   }
+  if (b->exec || b->dev) {
+    // the batch may have been launched on any stream
+    (void)hipDeviceSynchronize();
+  }
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
   }
   if (b->graph) {
     (void)hipGraphDestroy(b->graph);
   }
-  if (b->capture_stream) {
-    (void)hipStreamDestroy(b->capture_stream);
-  }
   if (b->dev) {
-    // the batch may have been launched on any stream
-    (void)hipDeviceSynchronize();
     (void)hipFree(b->dev);
   }
   delete b;

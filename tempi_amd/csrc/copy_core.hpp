@@ -119,7 +119,8 @@ __device__ __forceinline__ void copy_short(char *d, const char *s,
 }
 
 // byte offset of row `row` on the strided side
-__device__ __forceinline__ int64_t row_offset(const KParams &p, uint32_t row) {
+__host__ __device__ __forceinline__ int64_t row_offset(const KParams &p,
+                                                       uint32_t row) {
   int64_t off = 0;
   uint32_t q = row;
 #pragma unroll
@@ -138,6 +139,20 @@ __device__ __forceinline__ int64_t row_offset(const KParams &p, uint32_t row) {
   return off;
 }
 
+// How unit_addr moves the head and tail bytes of a misaligned row: the
+// kernels copy them (DeviceMem); the host-side emulation of a copy records
+// the bytes instead (kernels.hip), so one definition of the address
+// arithmetic serves both.
+struct DeviceMem {
+  __device__ static __forceinline__ void copy_short(char *d, const char *s,
+                                                    uint32_t n) {
+    ::tempi::copy_short(d, s, n);
+  }
+  __device__ static __forceinline__ void copy_byte(char *d, const char *s) {
+    *d = *s;
+  }
+};
+
 // Addresses of one work unit, computed before any of the thread's loads.
 struct UnitAddr {
   const char *s; // first body byte on the source side
@@ -149,9 +164,10 @@ struct UnitAddr {
 // Address phase of work unit `u` (wave-uniform when P == 64).  DIR 0 = pack
 // (strided -> packed), 1 = unpack (packed -> strided).  Head and tail bytes
 // of a row whose destination is not word-aligned are copied here directly.
-template <int P, int W, int U, bool ALIGNED, int DIR>
-__device__ __forceinline__ UnitAddr unit_addr(const KParams &p, uint32_t u,
-                                              uint32_t lane) {
+template <class M, int P, int W, int U, bool ALIGNED, int DIR>
+__host__ __device__ __forceinline__ UnitAddr unit_addr(const KParams &p,
+                                                       uint32_t u,
+                                                       uint32_t lane) {
   constexpr int A = word_align(W);
   const uint32_t row = p.nseg == 1 ? u : p.segdiv.div(u);
   const uint32_t seg = u - row * p.nseg;
@@ -175,14 +191,14 @@ __device__ __forceinline__ UnitAddr unit_addr(const KParams &p, uint32_t u,
       const uint32_t tail = (uint32_t)(p.L - t0);
       if constexpr (P == 1) {
         // one lane per row: the widest aligned stores, at most four each
-        copy_short(a.d, a.s, head);
-        copy_short(a.d + t0, a.s + t0, tail);
+        M::copy_short(a.d, a.s, head);
+        M::copy_short(a.d + t0, a.s + t0, tail);
       } else {
         for (uint32_t b = lane; b < head; b += P) {
-          a.d[b] = a.s[b];
+          M::copy_byte(a.d + b, a.s + b);
         }
         for (uint32_t b = lane; b < tail; b += P) {
-          a.d[t0 + b] = a.s[t0 + b];
+          M::copy_byte(a.d + t0 + b, a.s + t0 + b);
         }
       }
     }
@@ -228,7 +244,7 @@ __device__ __forceinline__ void copy_units(const KParams &p, uint32_t first,
       u = __builtin_amdgcn_readfirstlane(u);
     }
     if (u < end) {
-      a[k] = unit_addr<P, W, U, ALIGNED, DIR>(p, u, lane);
+      a[k] = unit_addr<DeviceMem, P, W, U, ALIGNED, DIR>(p, u, lane);
     } else {
       a[k].s = nullptr;
       a[k].d = nullptr;
@@ -290,8 +306,14 @@ constexpr int kBlock = 256;
   X(17, 4, 16, 2, false, true)  /* as 6, nontemporal loads             */     \
   X(18, 1, 8, 4, false, true)   /* as 8, nontemporal loads             */
 
-// Fill the kernel parameters of one copy with `variant`; `units` may not fit
-// 32 bits, which is reported as TEMPI_ERR_UNSUPPORTED.
+// Work units of one launch stay below 2^31.  The kernels index units with
+// 32-bit counters that step by at most (grid * groups * units in flight)
+// < 2^24 past the last unit, so no index wraps; launch_copy splits a larger
+// copy along its outer row dimensions.
+constexpr int64_t kMaxUnits = 1ll << 31;
+
+// Fill the kernel parameters of one copy with `variant`; a copy of
+// kMaxUnits units or more is reported as TEMPI_ERR_UNSUPPORTED.
 int fill_kparams(const Geom &g, const char *strided, char *packed, int dir,
                  int variant, KParams *p, bool *aligned);
 

@@ -114,6 +114,8 @@ struct Counters {
   std::atomic<int64_t> host_staged_calls{0}; // pack/unpack staged for pageable memory
 };
 Counters &counters();
+// log every counter, in rank order, at LOG_DEBUG (runtime.cpp)
+void report_counters();
 
 // While a TrialScope lives on a thread, the copies it launches are the
 // library's timing runs (verification, autotuning, system tables): they
@@ -240,10 +242,13 @@ struct Choice {
   int max_blocks = 0;
 };
 
-// variant selection entry used by the MPI and tempi_* paths; AUTOTUNE times
-// the candidates on the given buffers and stream (results cached by shape)
+// variant selection entry used by the MPI and tempi_* paths; AUTOTUNE (and
+// the default policy for copies of 64 MiB or more) times candidates on
+// `stream` (results cached by shape): on the caller's buffers when
+// `in_place` (the copy itself runs next, writing the same bytes), else on a
+// scratch image of the written side
 Choice choose(const Geom &g, const char *strided, char *packed, int dir,
-              int method, hipStream_t stream);
+              int method, hipStream_t stream, bool in_place);
 
 // grid caps the autotuner tries, as multiples of the CU count (0 = default)
 extern const int kGridCaps[];
@@ -287,6 +292,8 @@ bool times_cold_bytes(int64_t bytes);
 // next timed launch (autotune.hip)
 int flush_caches_on(int64_t bytes, hipStream_t s);
 constexpr int64_t kColdFlushBytesAll = 512ll << 20;
+// free the sweep buffer flush_caches_on allocated (autotune.hip)
+void flush_release();
 
 // Bytes from the first to the last byte a copy writes: the packed stream
 // for a pack, the span of the strided rows for an unpack.
@@ -321,8 +328,40 @@ uint64_t perf_generation();
 double perf_model_seconds(int method, bool colocated, int64_t bytes,
                           int64_t block_length);
 
-// the per-type record cache (interpose.cpp)
+// the per-type record cache (interpose.cpp); predefined types are lowered
+// on their first lookup
 bool type_lookup(MPI_Datatype t, Canon *out);
+// [lo, hi): the bytes, from the buffer origin, that `count` objects of `dt`
+// occupy (true lower bound to true upper bound over the objects)
+int type_span(MPI_Datatype dt, int count, int64_t *lo, int64_t *hi);
+// A pinned host image of the type-map span of `count` objects of `dt` in
+// device memory, for handing typed device data to an MPI library that
+// cannot read device memory (interpose.cpp).  take() copies the span in
+// when asked; put_back() copies the whole span back (so the bytes between
+// blocks keep the values they had when it was taken).
+struct SpanImage {
+  char *host = nullptr; // pinned host memory, `len` bytes
+  char *dev = nullptr;  // first byte of the span on the device
+  int64_t lo = 0;       // span start from the buffer origin
+  int64_t len = 0;
+  SpanImage() = default;
+  SpanImage(const SpanImage &) = delete;
+  SpanImage &operator=(const SpanImage &) = delete;
+  ~SpanImage();
+  int take(const void *dev_buf, MPI_Datatype dt, int count, bool copy_in);
+  int put_back();
+  // the buffer origin as the MPI library sees it
+  void *base() const { return host - lo; }
+};
+
+// Range of the calling thread's work for tracing tools (ROCTx), opened on
+// construction and closed on destruction (runtime.cpp).
+struct TraceRange {
+  explicit TraceRange(const char *name);
+  ~TraceRange();
+  TraceRange(const TraceRange &) = delete;
+  TraceRange &operator=(const TraceRange &) = delete;
+};
 // pack (dir 0) or unpack `count` objects on `stream` with the library's
 // variant choice (variant < 0) or the given one; device addresses
 int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,

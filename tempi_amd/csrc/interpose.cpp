@@ -36,13 +36,80 @@ int64_t key_of(MPI_Datatype t) { return (int64_t)(intptr_t)t; }
 
 bool type_lookup(MPI_Datatype t, Canon *out) {
   TypeCache &c = type_cache();
-  std::lock_guard<std::mutex> lk(c.mu);
-  auto it = c.map.find(key_of(t));
-  if (it == c.map.end()) {
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.map.find(key_of(t));
+    if (it != c.map.end()) {
+      *out = it->second.canon;
+      return true;
+    }
+  }
+  // Predefined types (MPI_FLOAT, MPI_BYTE, ...) are never committed: they
+  // are lowered on first use, so device data of such a type takes the
+  // device path instead of reaching the MPI library as a device pointer.
+  int ni = 0, na = 0, nd = 0, comb = 0, init = 0, fin = 0;
+  if (env().no_type_commit || t == MPI_DATATYPE_NULL ||
+      MPI_SUCCESS != MPI_Initialized(&init) || !init ||
+      MPI_SUCCESS != MPI_Finalized(&fin) || fin ||
+      MPI_SUCCESS != MPI_Type_get_envelope(t, &ni, &na, &nd, &comb) ||
+      comb != MPI_COMBINER_NAMED) {
     return false;
   }
-  *out = it->second.canon;
+  TypeRecord rec;
+  rec.canon = canonicalize(t);
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.map[key_of(t)] = rec;
+  *out = rec.canon;
   return true;
+}
+
+int type_span(MPI_Datatype dt, int count, int64_t *lo, int64_t *hi) {
+  MPI_Count tlb = 0, text = 0, lb = 0, ext = 0;
+  if (MPI_SUCCESS != MPI_Type_get_true_extent_x(dt, &tlb, &text) ||
+      MPI_SUCCESS != MPI_Type_get_extent_x(dt, &lb, &ext)) {
+    return set_error(TEMPI_ERR_ARG, "extent query failed");
+  }
+  const int64_t reach = (int64_t)std::max(count - 1, 0) * (int64_t)ext;
+  *lo = (int64_t)tlb + std::min<int64_t>(0, reach);
+  *hi = (int64_t)tlb + (int64_t)text + std::max<int64_t>(0, reach);
+  return TEMPI_SUCCESS;
+}
+
+SpanImage::~SpanImage() { pool_release(true, host); }
+
+int SpanImage::take(const void *dev_buf, MPI_Datatype dt, int count,
+                    bool copy_in) {
+  int64_t hi = 0;
+  int err = type_span(dt, count, &lo, &hi);
+  if (err) {
+    return err;
+  }
+  len = std::max<int64_t>(hi - lo, 0);
+  dev = static_cast<char *>(const_cast<void *>(dev_buf)) + lo;
+  if ((err = ensure_runtime())) {
+    return err;
+  }
+  host = static_cast<char *>(pool_acquire(true, (size_t)std::max<int64_t>(len, 1)));
+  if (!host) {
+    return set_error(TEMPI_ERR_NOMEM, "host image of %lld B", (long long)len);
+  }
+  if (copy_in && len > 0) {
+    hipStream_t s = library_stream();
+    TEMPI_HIP_CHECK(hipMemcpyAsync(host, dev, (size_t)len,
+                                   hipMemcpyDeviceToHost, s));
+    TEMPI_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  return TEMPI_SUCCESS;
+}
+
+int SpanImage::put_back() {
+  if (len > 0) {
+    hipStream_t s = library_stream();
+    TEMPI_HIP_CHECK(hipMemcpyAsync(dev, host, (size_t)len,
+                                   hipMemcpyHostToDevice, s));
+    TEMPI_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  return TEMPI_SUCCESS;
 }
 
 namespace {
@@ -64,7 +131,8 @@ int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
   }
   Choice c;
   if (variant < 0) {
-    c = choose(g, strided, packed, dir, env().select, stream);
+    c = choose(g, strided, packed, dir, env().select, stream,
+               /*in_place=*/true);
   } else {
     c.variant = variant;
   }
@@ -117,50 +185,82 @@ int pack_or_unpack(const Canon &cn, int count, const char *strided,
   return err;
 }
 
-// Host data and packed bytes in device memory the host cannot address: the
-// MPI library packs (or unpacks) the host side, as it does for host buffers,
-// and the packed bytes cross through pinned host memory.  Without this the
-// library would be handed a device pointer.  `bytes` packed bytes at
-// `packed`; MPI error code.
-int library_through_host(bool pack, const void *inbuf, int count,
-                         MPI_Datatype datatype, void *outbuf, int bytes,
-                         int *position, MPI_Comm comm) {
+// MPI_Pack / MPI_Unpack by the MPI library when one of the buffers is in
+// device memory the host cannot address and the library cannot read device
+// memory (it is not GPU-aware): the type has no strided form, the buffers
+// are host data on one side and device memory on the other, or packing is
+// switched off.  A device-resident typed buffer is replaced by a pinned
+// host image of its type map's span ([true lb, true ub) over the objects);
+// after an unpack the whole image is copied back, so the bytes between the
+// type map's blocks keep the values they had.  Device-resident packed
+// bytes cross through a pinned host buffer.  Without this the library would
+// be handed a device pointer.  MPI error code.
+int library_staged(bool pack, const void *typed, int count,
+                   MPI_Datatype datatype, void *packed, int packed_size,
+                   int *position, MPI_Comm comm) {
+  int bytes = 0;
+  if (MPI_SUCCESS != MPI_Pack_size(count, datatype, comm, &bytes) ||
+      *position < 0 || (int64_t)*position + bytes > (int64_t)packed_size) {
+    // the library's own size check would not see the staged buffers' sizes
+    MPI_Comm_call_errhandler(comm, MPI_ERR_TRUNCATE);
+    return MPI_ERR_TRUNCATE;
+  }
   if (ensure_runtime()) {
     return MPI_ERR_INTERN;
   }
-  void *stage = pool_acquire(true, (size_t)std::max(bytes, 1));
-  if (!stage) {
-    return MPI_ERR_NO_MEM;
-  }
   ++counters().host_staged_calls;
+  SpanImage image;
+  void *typed_at = const_cast<void *>(typed);
+  if (device_only(typed)) {
+    if (image.take(typed, datatype, count, /*copy_in=*/true)) {
+      TEMPI_LOG(LOG_ERROR, "staging for the MPI library: %s",
+                tempi_last_error());
+      return MPI_ERR_NO_MEM;
+    }
+    typed_at = image.base();
+  }
+  const bool packed_dev = device_only(packed);
+  HostStage stage;
+  char *packed_at = static_cast<char *>(packed) + *position;
   hipStream_t s = library_stream();
-  int err = MPI_SUCCESS;
-  int pos = 0;
-  if (pack) {
-    err = real_mpi().Pack(inbuf, count, datatype, stage, bytes, &pos, comm);
-    if (err == MPI_SUCCESS &&
-        (hipSuccess != hipMemcpyAsync(static_cast<char *>(outbuf) + *position,
-                                      stage, (size_t)pos,
-                                      hipMemcpyHostToDevice, s) ||
+  if (packed_dev) {
+    stage.p = pool_acquire(true, (size_t)std::max(bytes, 1));
+    if (!stage.p) {
+      return MPI_ERR_NO_MEM;
+    }
+    if (!pack &&
+        (hipSuccess != hipMemcpyAsync(stage.p, packed_at, (size_t)bytes,
+                                      hipMemcpyDeviceToHost, s) ||
          hipSuccess != hipStreamSynchronize(s))) {
-      err = MPI_ERR_INTERN;
+      return MPI_ERR_INTERN;
     }
-  } else {
-    if (hipSuccess != hipMemcpyAsync(stage,
-                                     static_cast<const char *>(inbuf) + *position,
-                                     (size_t)bytes, hipMemcpyDeviceToHost, s) ||
-        hipSuccess != hipStreamSynchronize(s)) {
-      err = MPI_ERR_INTERN;
-    } else {
-      err = real_mpi().Unpack(stage, bytes, &pos, outbuf, count, datatype,
-                              comm);
-    }
+    packed_at = static_cast<char *>(stage.p);
   }
-  pool_release(true, stage);
-  if (err == MPI_SUCCESS) {
-    *position += pos;
+  int pos = 0;
+  int err = pack ? real_mpi().Pack(typed_at, count, datatype, packed_at, bytes,
+                                   &pos, comm)
+                 : real_mpi().Unpack(packed_at, bytes, &pos, typed_at, count,
+                                     datatype, comm);
+  if (err != MPI_SUCCESS) {
+    return err;
   }
-  return err;
+  if (pack && packed_dev &&
+      (hipSuccess != hipMemcpyAsync(static_cast<char *>(packed) + *position,
+                                    stage.p, (size_t)pos,
+                                    hipMemcpyHostToDevice, s) ||
+       hipSuccess != hipStreamSynchronize(s))) {
+    return MPI_ERR_INTERN;
+  }
+  if (!pack && image.host && image.put_back()) {
+    return MPI_ERR_INTERN;
+  }
+  *position += pos;
+  return MPI_SUCCESS;
+}
+
+// would the MPI library be handed device memory it cannot address?
+bool library_needs_staging(const void *a, const void *b) {
+  return !env().gpu_aware_mpi && (device_only(a) || device_only(b));
 }
 
 } // namespace tempi
@@ -204,6 +304,7 @@ TEMPI_EXPORT int MPI_Init_thread(int *argc, char ***argv, int required,
 TEMPI_EXPORT int MPI_Finalize(void) {
   ensure_env();
   if (!env().disable) {
+    report_counters();
     {
       TypeCache &c = type_cache();
       std::lock_guard<std::mutex> lk(c.mu);
@@ -262,25 +363,21 @@ TEMPI_EXPORT int MPI_Pack(const void *inbuf, int incount, MPI_Datatype datatype,
                           MPI_Comm comm) {
   ensure_env();
   const RealMpi &lib = real_mpi();
+  if (env().disable) {
+    return lib.Pack(inbuf, incount, datatype, outbuf, outsize, position, comm);
+  }
+  TraceRange trace("MPI_Pack");
   Canon cn;
   void *din = nullptr, *dout = nullptr;
   // device data packed into memory the device cannot address (pageable host
   // memory) is staged on the device; the reference hands that case to the
   // MPI library, which cannot read device memory
-  if (env().disable || env().no_pack || incount <= 0 || !position ||
-      !outbuf || !lookup(datatype, &cn) || !cn.ok ||
-      !device_address(inbuf, &din)) {
-    if (!env().disable && incount > 0 && position && *position >= 0 &&
-        inbuf && !device_address(inbuf, &din) && device_only(outbuf)) {
-      // host data packed into device memory
-      int size = 0;
-      if (MPI_SUCCESS != MPI_Pack_size(incount, datatype, comm, &size) ||
-          (int64_t)*position + size > (int64_t)outsize) {
-        MPI_Comm_call_errhandler(comm, MPI_ERR_TRUNCATE);
-        return MPI_ERR_TRUNCATE;
-      }
-      return library_through_host(true, inbuf, incount, datatype, outbuf, size,
-                                  position, comm);
+  if (env().no_pack || incount <= 0 || !position || !outbuf ||
+      !lookup(datatype, &cn) || !cn.ok || !device_address(inbuf, &din)) {
+    if (incount > 0 && position && inbuf && outbuf &&
+        library_needs_staging(inbuf, outbuf)) {
+      return library_staged(true, inbuf, incount, datatype, outbuf, outsize,
+                            position, comm);
     }
     ++counters().library_pack_calls;
     return lib.Pack(inbuf, incount, datatype, outbuf, outsize, position, comm);
@@ -314,24 +411,21 @@ TEMPI_EXPORT int MPI_Unpack(const void *inbuf, int insize, int *position,
                             MPI_Comm comm) {
   ensure_env();
   const RealMpi &lib = real_mpi();
+  if (env().disable) {
+    return lib.Unpack(inbuf, insize, position, outbuf, outcount, datatype,
+                      comm);
+  }
+  TraceRange trace("MPI_Unpack");
   Canon cn;
   void *din = nullptr, *dout = nullptr;
   // packed bytes in pageable host memory unpacked into device data: staged
   // on the device, as in MPI_Pack
-  if (env().disable || env().no_pack || outcount <= 0 || !position ||
-      !inbuf || !lookup(datatype, &cn) || !cn.ok ||
-      !device_address(outbuf, &dout)) {
-    if (!env().disable && outcount > 0 && position && *position >= 0 &&
-        outbuf && !device_address(outbuf, &dout) && device_only(inbuf)) {
-      // packed bytes in device memory unpacked into host data
-      int size = 0;
-      if (MPI_SUCCESS != MPI_Pack_size(outcount, datatype, comm, &size) ||
-          (int64_t)*position + size > (int64_t)insize) {
-        MPI_Comm_call_errhandler(comm, MPI_ERR_TRUNCATE);
-        return MPI_ERR_TRUNCATE;
-      }
-      return library_through_host(false, inbuf, outcount, datatype, outbuf,
-                                  size, position, comm);
+  if (env().no_pack || outcount <= 0 || !position || !inbuf ||
+      !lookup(datatype, &cn) || !cn.ok || !device_address(outbuf, &dout)) {
+    if (outcount > 0 && position && inbuf && outbuf &&
+        library_needs_staging(inbuf, outbuf)) {
+      return library_staged(false, outbuf, outcount, datatype,
+                            const_cast<void *>(inbuf), insize, position, comm);
     }
     ++counters().library_unpack_calls;
     return lib.Unpack(inbuf, insize, position, outbuf, outcount, datatype,
@@ -455,13 +549,13 @@ TEMPI_EXPORT int tempi_select_variant(const tempi_strided_block *sb,
       return set_error(TEMPI_ERR_ARG, "autotuning needs device-accessible buffers");
     }
     *variant = choose(g, static_cast<const char *>(ds), static_cast<char *>(dp),
-                      direction, method, library_stream())
+                      direction, method, library_stream(), /*in_place=*/false)
                    .variant;
     return TEMPI_SUCCESS;
   }
   *variant = choose(g, static_cast<const char *>(strided_buf),
                     static_cast<char *>(const_cast<void *>(packed_buf)),
-                    direction, method, nullptr)
+                    direction, method, nullptr, /*in_place=*/false)
                  .variant;
   return TEMPI_SUCCESS;
 }

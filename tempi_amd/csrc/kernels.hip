@@ -208,9 +208,9 @@ int fill_kparams(const Geom &g, const char *strided, char *packed, int dir,
   const int64_t per_unit = (int64_t)v.P * v.U;
   const int64_t nseg = std::max<int64_t>(1, (words + per_unit - 1) / per_unit);
   const int64_t units = g.rows * nseg;
-  if (g.rows >= (1ll << 32) || units >= (1ll << 32) || words >= (1ll << 32)) {
+  if (units >= kMaxUnits || words >= (1ll << 32)) {
     return set_error(TEMPI_ERR_UNSUPPORTED,
-                     "copy of %lld rows of %lld B exceeds the 2^32 unit space",
+                     "copy of %lld rows of %lld B exceeds 2^31 work units",
                      (long long)g.rows, (long long)g.L);
   }
   KParams &p = *pp;
@@ -240,6 +240,94 @@ int fill_kparams(const Geom &g, const char *strided, char *packed, int dir,
   return TEMPI_SUCCESS;
 }
 
+// workgroups of one launch of `units` units
+static int64_t grid_of(int variant, uint32_t units, int max_blocks,
+                       int cu_count) {
+  const int64_t groups = (int64_t)(kBlock / kVariants[variant].P) *
+                         units_in_flight(kVariants[variant].U);
+  int64_t blocks = ((int64_t)units + groups - 1) / groups;
+  const int64_t cap = max_blocks > 0 ? max_blocks : (int64_t)cu_count * 16;
+  blocks = std::min<int64_t>(blocks, cap);
+  return std::max<int64_t>(blocks, 1);
+}
+
+// segments per row of `variant` (the unit count is rows times this)
+static int64_t segments_of(const Geom &g, int variant) {
+  const VariantDesc &v = kVariants[variant];
+  const int64_t per_unit = (int64_t)v.P * v.U;
+  return std::max<int64_t>(1, (g.L / v.W + per_unit - 1) / per_unit);
+}
+
+// Call piece(geometry, strided, packed) for each launch of a copy: rows
+// longer than 1 GiB go as 1 GiB pieces, and a copy of kMaxUnits units or
+// more is cut along its outer row dimensions -- a run of whole indices of
+// the outermost dimension per launch, or one launch per index when a single
+// index already holds too many units.
+template <class F>
+static int split_units(const Geom &g, const char *strided, char *packed,
+                       int64_t nseg, F &piece) {
+  if (g.rows * nseg < kMaxUnits || g.nrd == 0) {
+    return piece(g, strided, packed);
+  }
+  const int k = g.nrd - 1;
+  const int64_t c = g.cnt[k];
+  const int64_t inner = g.rows / c;
+  Geom sub = g;
+  if (inner * nseg < kMaxUnits) {
+    const int64_t m = (kMaxUnits - 1) / (inner * nseg);
+    for (int64_t o = 0; o < c; o += m) {
+      const int64_t n = std::min(m, c - o);
+      sub.cnt[k] = n;
+      sub.rows = inner * n;
+      sub.start = g.start + o * g.str[k];
+      int err = piece(sub, strided, packed + o * inner * g.L);
+      if (err) {
+        return err;
+      }
+    }
+    return TEMPI_SUCCESS;
+  }
+  sub.nrd = k;
+  sub.rows = inner;
+  for (int64_t o = 0; o < c; ++o) {
+    sub.start = g.start + o * g.str[k];
+    int err = split_units(sub, strided, packed + o * inner * g.L, nseg, piece);
+    if (err) {
+      return err;
+    }
+  }
+  return TEMPI_SUCCESS;
+}
+
+template <class F>
+static int for_each_launch(const Geom &g, const char *strided, char *packed,
+                           int variant, F &&piece) {
+  constexpr int64_t kPiece = 1ll << 30;
+  if (g.L <= kPiece) {
+    return split_units(g, strided, packed, segments_of(g, variant), piece);
+  }
+  Geom row = g;
+  row.nrd = 0;
+  row.rows = 1;
+  for (int64_t r = 0; r < g.rows; ++r) {
+    // strided offset of row r
+    int64_t off = g.start, q = r;
+    for (int k = 0; k < g.nrd; ++k) {
+      off += (q % g.cnt[k]) * g.str[k];
+      q /= g.cnt[k];
+    }
+    for (int64_t x = 0; x < g.L; x += kPiece) {
+      row.L = std::min(kPiece, g.L - x);
+      row.start = off + x;
+      int err = piece(row, strided, packed + r * g.L + x);
+      if (err) {
+        return err;
+      }
+    }
+  }
+  return TEMPI_SUCCESS;
+}
+
 static int launch_one(const Geom &g, const char *strided, char *packed,
                       int dir, int variant, hipStream_t stream,
                       int max_blocks) {
@@ -249,13 +337,7 @@ static int launch_one(const Geom &g, const char *strided, char *packed,
   if (err) {
     return err;
   }
-  const int64_t groups = (int64_t)(kBlock / kVariants[variant].P) *
-                         units_in_flight(kVariants[variant].U);
-  int64_t blocks = ((int64_t)p.units + groups - 1) / groups;
-  const int64_t cap =
-      max_blocks > 0 ? max_blocks : (int64_t)device_cu_count() * 16;
-  blocks = std::min<int64_t>(blocks, cap);
-  blocks = std::max<int64_t>(blocks, 1);
+  const int64_t blocks = grid_of(variant, p.units, max_blocks, device_cu_count());
   KernelFn fn = kKernels[variant][aligned ? 1 : 0][dir];
   void *args[] = {&p};
   TEMPI_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void *>(fn),
@@ -287,33 +369,233 @@ int launch_copy(const Geom &g, const char *strided, char *packed, int dir,
   if (variant < 0 || variant >= kNumVariants) {
     return set_error(TEMPI_ERR_ARG, "variant %d out of range", variant);
   }
-  // rows longer than 1 GiB are moved as 1 GiB pieces, one launch each
-  constexpr int64_t kPiece = 1ll << 30;
-  if (g.L <= kPiece) {
-    return launch_one(g, strided, packed, dir, variant, stream, max_blocks);
+  return for_each_launch(g, strided, packed, variant,
+                         [&](const Geom &pg, const char *ps, char *pp) {
+                           return launch_one(pg, ps, pp, dir, variant, stream,
+                                             max_blocks);
+                         });
+}
+
+// ---------------------------------------------------------------------------
+// host-side emulation of a copy's memory accesses
+// ---------------------------------------------------------------------------
+//
+// Runs the launches, grid, grid-stride index sequence and unit addresses
+// that launch_copy and the kernels would, with the same code for the
+// address arithmetic (unit_addr), and records every byte range loaded and
+// stored instead of touching memory.  The buffer addresses are numbers
+// only; nothing is dereferenced.  It shows on the CPU which bytes a launch
+// reads and writes for any variant, alignment and shape.
+
+namespace {
+
+struct Recorder {
+  uintptr_t rbase = 0, wbase = 0; // origins of the read and written sides
+  int64_t rlo = INT64_MAX, rhi = INT64_MIN, wlo = INT64_MAX, whi = INT64_MIN;
+  int64_t nread = 0, nwritten = 0;
+  // per-byte store counts over written-side offsets [map_lo, map_lo + size)
+  std::vector<uint8_t> map;
+  int64_t map_lo = 0;
+  bool outside_map = false;
+  void read(const char *p, int64_t n) {
+    const int64_t o = (int64_t)((uintptr_t)p - rbase);
+    rlo = std::min(rlo, o);
+    rhi = std::max(rhi, o + n);
+    nread += n;
   }
-  Geom piece = g;
-  piece.nrd = 0;
-  piece.rows = 1;
-  for (int64_t r = 0; r < g.rows; ++r) {
-    // strided offset of row r
-    int64_t off = g.start, q = r;
-    for (int k = 0; k < g.nrd; ++k) {
-      off += (q % g.cnt[k]) * g.str[k];
-      q /= g.cnt[k];
-    }
-    for (int64_t x = 0; x < g.L; x += kPiece) {
-      piece.L = std::min(kPiece, g.L - x);
-      piece.start = off + x;
-      int err = launch_one(piece, strided, packed + r * g.L + x, dir, variant,
-                           stream, max_blocks);
-      if (err) {
-        return err;
+  void write(const char *p, int64_t n) {
+    const int64_t o = (int64_t)((uintptr_t)p - wbase);
+    wlo = std::min(wlo, o);
+    whi = std::max(whi, o + n);
+    nwritten += n;
+    if (!map.empty()) {
+      for (int64_t i = o; i < o + n; ++i) {
+        const int64_t j = i - map_lo;
+        if (j < 0 || j >= (int64_t)map.size()) {
+          outside_map = true;
+        } else if (map[(size_t)j] < 255) {
+          ++map[(size_t)j];
+        }
       }
     }
   }
+};
+
+thread_local Recorder *t_rec = nullptr;
+
+struct HostMem {
+  static void copy_short(char *d, const char *s, uint32_t n) {
+    t_rec->read(s, n);
+    t_rec->write(d, n);
+  }
+  static void copy_byte(char *d, const char *s) {
+    t_rec->read(s, 1);
+    t_rec->write(d, 1);
+  }
+};
+
+// one work unit of one group, all P lanes: the kernels' address phase, then
+// the body words of the load and store phases
+template <int P, int W, int U, bool ALIGNED, int DIR>
+void emulate_unit(const KParams &p, uint32_t u) {
+  for (uint32_t lane = 0; lane < (uint32_t)P; ++lane) {
+    const UnitAddr a = unit_addr<HostMem, P, W, U, ALIGNED, DIR>(p, u, lane);
+    for (int j = 0; j < U; ++j) {
+      const uint32_t w = a.w + (uint32_t)(j * P);
+      if (w < a.n) {
+        t_rec->read(a.s + (size_t)w * W, W);
+        t_rec->write(a.d + (size_t)w * W, W);
+      }
+    }
+  }
+}
+
+typedef void (*EmulateFn)(const KParams &, uint32_t);
+#define TEMPI_EROW(I, P, W, U, NTS, NTL)                                         \
+  {{emulate_unit<P, W, U, false, 0>, emulate_unit<P, W, U, false, 1>},          \
+   {emulate_unit<P, W, U, true, 0>, emulate_unit<P, W, U, true, 1>}},
+const EmulateFn kEmulate[][2][2] = {TEMPI_VARIANTS(TEMPI_EROW)};
+#undef TEMPI_EROW
+
+} // namespace
+
+} // namespace tempi
+
+using namespace tempi;
+
+// Emulates launch_copy of the copy on the host (see above); `unit_limit`
+// bounds the work units emulated in full (beyond it only the launch plan is
+// reported).
+TEMPI_EXPORT int tempi_copy_emulate(const tempi_strided_block *sb,
+                                    int64_t incount, uint64_t strided_addr,
+                                    uint64_t packed_addr, int direction,
+                                    int variant, int max_blocks, int cu_count,
+                                    int64_t unit_limit,
+                                    tempi_copy_emulation *out) {
+  if (!sb || !out || cu_count <= 0 || max_blocks < 0) {
+    return set_error(TEMPI_ERR_ARG, "bad argument");
+  }
+  int err = check_direction(direction);
+  if (err) {
+    return err;
+  }
+  if (variant < 0 || variant >= kNumVariants) {
+    return set_error(TEMPI_ERR_ARG, "variant %d out of range", variant);
+  }
+  Geom g;
+  if ((err = make_geom(*sb, incount, &g))) {
+    return err;
+  }
+  *out = tempi_copy_emulation{};
+  const char *strided = reinterpret_cast<const char *>((uintptr_t)strided_addr);
+  char *packed = reinterpret_cast<char *>((uintptr_t)packed_addr);
+  if (g.rows == 0 || g.L == 0) {
+    out->exact = 1;
+    return TEMPI_SUCCESS;
+  }
+  // plan: launches and units
+  int64_t units_total = 0;
+  err = for_each_launch(g, strided, packed, variant,
+                        [&](const Geom &pg, const char *, char *) {
+                          const int64_t u = pg.rows * segments_of(pg, variant);
+                          ++out->launches;
+                          out->max_units = std::max(out->max_units, u);
+                          units_total += u;
+                          return TEMPI_SUCCESS;
+                        });
+  if (err) {
+    return err;
+  }
+  if (units_total > unit_limit) {
+    out->emulated = 0;
+    return TEMPI_SUCCESS;
+  }
+  Recorder rec;
+  const bool pack = direction == TEMPI_DIR_PACK;
+  rec.rbase = pack ? (uintptr_t)strided : (uintptr_t)packed;
+  rec.wbase = pack ? (uintptr_t)packed : (uintptr_t)strided;
+  // the bytes the copy must write, each once: the packed stream, or the
+  // strided rows (a map of the written side, up to 256 MiB)
+  int64_t lo = 0, hi = g.L;
+  for (int k = 0; k < g.nrd; ++k) {
+    const int64_t reach = (g.cnt[k] - 1) * g.str[k];
+    (reach < 0 ? lo : hi) += reach;
+  }
+  const int64_t map_lo = pack ? 0 : g.start + lo;
+  const int64_t map_len = pack ? g.rows * g.L : hi - lo;
+  std::vector<uint8_t> want;
+  if (map_len <= (256ll << 20)) {
+    rec.map.assign((size_t)map_len, 0);
+    rec.map_lo = map_lo;
+    want.assign((size_t)map_len, 0);
+    for (int64_t r = 0; r < g.rows; ++r) {
+      int64_t off = pack ? r * g.L : g.start, q = r;
+      for (int k = 0; !pack && k < g.nrd; ++k) {
+        off += (q % g.cnt[k]) * g.str[k];
+        q /= g.cnt[k];
+      }
+      for (int64_t x = 0; x < g.L; ++x) {
+        uint8_t &c = want[(size_t)(off + x - map_lo)];
+        c = (uint8_t)std::min(255, c + 1);
+      }
+    }
+  }
+  t_rec = &rec;
+  err = for_each_launch(
+      g, strided, packed, variant,
+      [&](const Geom &pg, const char *ps, char *pp) -> int {
+        KParams p;
+        bool aligned = false;
+        int e = fill_kparams(pg, ps, pp, direction, variant, &p, &aligned);
+        if (e) {
+          return e;
+        }
+        const VariantDesc &v = kVariants[variant];
+        const EmulateFn fn = kEmulate[variant][aligned ? 1 : 0][direction];
+        const uint32_t GROUPS = (uint32_t)(kBlock / v.P);
+        const uint32_t K = (uint32_t)units_in_flight(v.U);
+        const uint32_t blocks =
+            (uint32_t)grid_of(variant, p.units, max_blocks, cu_count);
+        const uint32_t step = blocks * GROUPS;
+        // the kernels' index sequence, in their 32-bit arithmetic
+        for (uint32_t b = 0; b < blocks; ++b) {
+          for (uint32_t grp = 0; grp < GROUPS; ++grp) {
+            for (uint32_t u = b * GROUPS + grp; u < p.units;) {
+              for (uint32_t k = 0; k < K; ++k) {
+                const uint32_t uu = u + k * step;
+                if ((uint64_t)u + (uint64_t)k * step != (uint64_t)uu) {
+                  out->wrapped = 1;
+                } else if (uu < p.units) {
+                  fn(p, uu);
+                }
+              }
+              const uint64_t next = (uint64_t)u + (uint64_t)step * K;
+              if (next >= (1ull << 32)) {
+                out->wrapped = 1;
+                break;
+              }
+              u = (uint32_t)next;
+            }
+          }
+        }
+        return TEMPI_SUCCESS;
+      });
+  t_rec = nullptr;
+  if (err) {
+    return err;
+  }
+  out->emulated = 1;
+  out->read_lo = rec.nread ? rec.rlo : 0;
+  out->read_hi = rec.nread ? rec.rhi : 0;
+  out->write_lo = rec.nwritten ? rec.wlo : 0;
+  out->write_hi = rec.nwritten ? rec.whi : 0;
+  out->bytes_read = rec.nread;
+  out->bytes_written = rec.nwritten;
+  out->exact = rec.map.empty() ? -1 : (!rec.outside_map && rec.map == want);
   return TEMPI_SUCCESS;
 }
+
+namespace tempi {
 
 // exported for the host-side unit test of the divisor arithmetic
 extern "C" __attribute__((visibility("default"))) uint32_t

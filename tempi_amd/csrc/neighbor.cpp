@@ -305,10 +305,25 @@ TEMPI_EXPORT int MPI_Neighbor_alltoallw(
       return run(*g_entries.front(), comm);
     }
   }
+  // device buffers the MPI library cannot read: the exchange runs here or
+  // fails with an error, never with device pointers in the library
+  const bool device_only_bufs =
+      !env().gpu_aware_mpi && ((outdeg > 0 && device_only(sendbuf)) ||
+                               (indeg > 0 && device_only(recvbuf)));
+  auto unsupported = [&](const char *why) {
+    set_error(TEMPI_ERR_UNSUPPORTED,
+              "MPI_Neighbor_alltoallw of device buffers: %s", why);
+    TEMPI_LOG(LOG_ERROR, "%s", tempi_last_error());
+    MPI_Comm_call_errhandler(comm, MPI_ERR_TYPE);
+    return MPI_ERR_TYPE;
+  };
   std::vector<Canon> scn, rcn;
   auto e = std::make_unique<Exchange>();
   if (!lower_side(outdeg, sendcounts, sendtypes, &scn, &e->out) ||
       !lower_side(indeg, recvcounts, recvtypes, &rcn, &e->in)) {
+    if (device_only_bufs) {
+      return unsupported("a type has no strided form");
+    }
     return library();
   }
   e->comm = comm;
@@ -324,6 +339,9 @@ TEMPI_EXPORT int MPI_Neighbor_alltoallw(
   all.insert(all.end(), rcn.begin(), rcn.end());
   e->method = pick_method(std::max(e->out.total, e->in.total), all);
   if (build(*e, outdeg, indeg, scn, rcn)) {
+    if (device_only_bufs) {
+      return unsupported(tempi_last_error());
+    }
     TEMPI_LOG(LOG_WARN, "MPI_Neighbor_alltoallw: %s; library path",
               tempi_last_error());
     return library();

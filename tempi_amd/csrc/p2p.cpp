@@ -261,9 +261,93 @@ public:
   // block until the packed bytes are handed to the library (sends)
   virtual int post_blocking() { return MPI_SUCCESS; }
   virtual bool done() const = 0;
+  // complete if possible without blocking; *flag = completed
+  virtual int test(int *flag, MPI_Status *st) = 0;
   // block until complete
   virtual int wait(MPI_Status *st) = 0;
 };
+
+// Typed device data whose type has no strided form, handed to an MPI
+// library that cannot read device memory: the library sends from (or
+// receives into) a pinned host image of the type map's span, taken at the
+// start; a receive copies the image back when it completes, so the bytes
+// between the type map's blocks keep their values.
+class SpanOp : public AsyncOp {
+  bool send_;
+  bool done_ = false;
+  SpanImage img_;
+  MPI_Request lib_req_ = MPI_REQUEST_NULL;
+  MPI_Status status_{};
+  int error_ = MPI_SUCCESS;
+
+  int finish() {
+    done_ = true;
+    if (!send_ && error_ == MPI_SUCCESS && img_.put_back()) {
+      TEMPI_LOG(LOG_ERROR, "receive into device memory: %s", tempi_last_error());
+      error_ = MPI_ERR_INTERN;
+    }
+    return error_;
+  }
+
+public:
+  explicit SpanOp(bool send) : send_(send) {}
+  ~SpanOp() override {
+    if (!done_ && lib_req_ != MPI_REQUEST_NULL) {
+      (void)real_mpi().Wait(&lib_req_, MPI_STATUS_IGNORE);
+    }
+  }
+  int start(const void *buf, int count, MPI_Datatype dt, int peer, int tag,
+            MPI_Comm comm) {
+    if (img_.take(buf, dt, count, /*copy_in=*/true)) {
+      TEMPI_LOG(LOG_ERROR, "staging device data: %s", tempi_last_error());
+      return MPI_ERR_NO_MEM;
+    }
+    return send_ ? real_mpi().Isend(img_.base(), count, dt, peer, tag, comm,
+                                    &lib_req_)
+                 : real_mpi().Irecv(img_.base(), count, dt, peer, tag, comm,
+                                    &lib_req_);
+  }
+  bool is_send() const override { return send_; }
+  int wake() override {
+    if (done_) {
+      return MPI_SUCCESS;
+    }
+    int flag = 0;
+    const int err = real_mpi().Test(&lib_req_, &flag, &status_);
+    if (err != MPI_SUCCESS) {
+      error_ = err;
+    }
+    return flag || err != MPI_SUCCESS ? finish() : MPI_SUCCESS;
+  }
+  bool done() const override { return done_; }
+  int test(int *flag, MPI_Status *st) override {
+    const int err = wake();
+    *flag = done_;
+    if (done_ && st != MPI_STATUS_IGNORE) {
+      *st = status_;
+    }
+    return err;
+  }
+  int wait(MPI_Status *st) override {
+    if (!done_) {
+      const int err = real_mpi().Wait(&lib_req_, &status_);
+      if (err != MPI_SUCCESS) {
+        error_ = err;
+      }
+      finish();
+    }
+    if (st != MPI_STATUS_IGNORE) {
+      *st = status_;
+    }
+    return error_;
+  }
+};
+
+// Would the MPI library be handed `buf` in device memory it cannot read?
+bool library_cannot_read(const void *buf, int count, int peer) {
+  return !env().disable && !env().gpu_aware_mpi && count > 0 &&
+         peer != MPI_PROC_NULL && buf && device_only(buf);
+}
 
 class IsendOp : public AsyncOp {
   enum { PACKING, SENDING, DONE } state_ = PACKING;
@@ -334,7 +418,7 @@ public:
     return post();
   }
   bool done() const override { return state_ == DONE; }
-  int test(int *flag, MPI_Status *st) {
+  int test(int *flag, MPI_Status *st) override {
     *flag = 0;
     int err = wake();
     if (err || state_ != SENDING) {
@@ -435,7 +519,7 @@ public:
     return MPI_SUCCESS;
   }
   bool done() const override { return state_ == DONE; }
-  int test(int *flag, MPI_Status *st) {
+  int test(int *flag, MPI_Status *st) override {
     int err = wake();
     if (!err && state_ == UNPACKING) {
       err = wake();
@@ -560,7 +644,9 @@ bool device_path(const void *buf, int count, MPI_Datatype dt, int peer,
   if (contiguous) {
     // contiguous device data goes to the library as it is unless staging is
     // asked for (reference: ContiguousMethod, src/internal/env.cpp:81-86)
-    if (env().contiguous_method == TEMPI_METHOD_STAGED) {
+    // or the library cannot read device memory
+    if (env().contiguous_method == TEMPI_METHOD_STAGED ||
+        (!env().gpu_aware_mpi && device_only(buf))) {
       *method_out = TEMPI_METHOD_STAGED;
       return true;
     }
@@ -574,6 +660,22 @@ bool device_path(const void *buf, int count, MPI_Datatype dt, int peer,
   }
   *method_out = TEMPI_METHOD_UNKNOWN; // chosen by the caller with the peer
   return true;
+}
+
+// a non-blocking transfer of device data the MPI library cannot read, from
+// or into a host image of its span (SpanOp)
+int start_span_op(bool send, const void *buf, int count, MPI_Datatype dt,
+                  int peer, int tag, MPI_Comm comm, MPI_Request *request) {
+  auto op = std::make_unique<SpanOp>(send);
+  const int err = op->start(buf, count, dt, peer, tag, comm);
+  if (err != MPI_SUCCESS) {
+    return err;
+  }
+  ++counters().host_staged_calls;
+  *request = mint_handle(op.get());
+  g_active[key_of(*request)] = std::move(op);
+  (void)try_progress();
+  return MPI_SUCCESS;
 }
 
 int check_count(int count, const Canon &cn, int64_t *bytes) {
@@ -682,6 +784,15 @@ TEMPI_EXPORT int MPI_Send(const void *buf, int count, MPI_Datatype datatype,
       ++counters().p2p_library;
       (void)post_pending_sends();
     }
+    if (library_cannot_read(buf, count, dest)) {
+      SpanImage img;
+      if (img.take(buf, datatype, count, /*copy_in=*/true)) {
+        TEMPI_LOG(LOG_ERROR, "MPI_Send: %s", tempi_last_error());
+        return MPI_ERR_NO_MEM;
+      }
+      ++counters().host_staged_calls;
+      return real_mpi().Send(img.base(), count, datatype, dest, tag, comm);
+    }
     return real_mpi().Send(buf, count, datatype, dest, tag, comm);
   }
   int64_t bytes = 0;
@@ -727,6 +838,21 @@ TEMPI_EXPORT int MPI_Recv(void *buf, int count, MPI_Datatype datatype,
     if (!env().disable) {
       ++counters().p2p_library;
       (void)post_pending_sends();
+    }
+    if (library_cannot_read(buf, count, source)) {
+      SpanImage img;
+      if (img.take(buf, datatype, count, /*copy_in=*/true)) {
+        TEMPI_LOG(LOG_ERROR, "MPI_Recv: %s", tempi_last_error());
+        return MPI_ERR_NO_MEM;
+      }
+      ++counters().host_staged_calls;
+      int err = real_mpi().Recv(img.base(), count, datatype, source, tag, comm,
+                                status);
+      if (err == MPI_SUCCESS && img.put_back()) {
+        TEMPI_LOG(LOG_ERROR, "MPI_Recv: %s", tempi_last_error());
+        err = MPI_ERR_INTERN;
+      }
+      return err;
     }
     return real_mpi().Recv(buf, count, datatype, source, tag, comm, status);
   }
@@ -782,6 +908,9 @@ TEMPI_EXPORT int MPI_Isend(const void *buf, int count, MPI_Datatype datatype,
     if (!env().disable) {
       ++counters().p2p_library;
     }
+    if (request && library_cannot_read(buf, count, dest)) {
+      return start_span_op(true, buf, count, datatype, dest, tag, comm, request);
+    }
     const int err = real_mpi().Isend(buf, count, datatype, dest, tag, comm, request);
     if (!env().disable) {
       (void)try_progress();
@@ -821,6 +950,10 @@ TEMPI_EXPORT int MPI_Irecv(void *buf, int count, MPI_Datatype datatype,
       !device_path(buf, count, datatype, source, &cn, &dev, &method)) {
     if (!env().disable) {
       ++counters().p2p_library;
+    }
+    if (request && library_cannot_read(buf, count, source)) {
+      return start_span_op(false, buf, count, datatype, source, tag, comm,
+                           request);
     }
     const int err = real_mpi().Irecv(buf, count, datatype, source, tag, comm, request);
     if (!env().disable) {
@@ -926,12 +1059,7 @@ TEMPI_EXPORT int MPI_Test(MPI_Request *request, int *flag, MPI_Status *status) {
   if (!op) {
     return real_mpi().Test(request, flag, status);
   }
-  int err = MPI_SUCCESS;
-  if (op->is_send()) {
-    err = static_cast<IsendOp *>(op)->test(flag, status);
-  } else {
-    err = static_cast<IrecvOp *>(op)->test(flag, status);
-  }
+  const int err = op->test(flag, status);
   if (*flag) {
     g_active.erase(key_of(*request));
     *request = MPI_REQUEST_NULL;

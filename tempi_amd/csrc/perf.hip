@@ -766,7 +766,8 @@ int measure_table(int table, double max_s, Table *out) {
       }
       const char *sp = static_cast<const char *>(strided.p);
       char *pp = static_cast<char *>(packed_dev);
-      const Choice ch = choose(g, sp, pp, dir, env().select, s);
+      const Choice ch = choose(g, sp, pp, dir, env().select, s,
+                               /*in_place=*/true);
       IidTime r;
       err = sample_point(
           [&](double *tm) -> int {

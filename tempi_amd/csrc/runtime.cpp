@@ -9,10 +9,64 @@
 #include "internal.hpp"
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <atomic>
 
 namespace tempi {
+
+// ROCTx ranges (reference: the NVTX ranges around MPI_Pack and MPI_Unpack,
+// src/pack.cpp:37,66 and src/unpack.cpp:29,58).  Without a tracing tool
+// attached a push or pop returns at once.
+TraceRange::TraceRange(const char *name) { roctxRangePushA(name); }
+TraceRange::~TraceRange() { roctxRangePop(); }
+
+// The counters, one line each, in rank order at MPI_Finalize when the log
+// level is debug or higher (reference: counters::finalize,
+// src/internal/counters.cpp:30-120, which prints at LOG_DEBUG).
+void report_counters() {
+  if (g_log_level < LOG_DEBUG) {
+    return;
+  }
+  tempi_counters c{};
+  (void)tempi_counters_get(&c);
+  int rank = 0, size = 1, init = 0;
+  if (MPI_SUCCESS == MPI_Initialized(&init) && init) {
+    MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+    MPI_Comm_size(MPI_COMM_WORLD, &size);
+  }
+  const std::pair<const char *, int64_t> rows[] = {
+      {"pack_kernel_launches", c.pack_kernel_launches},
+      {"unpack_kernel_launches", c.unpack_kernel_launches},
+      {"batch_launches", c.batch_launches},
+      {"graph_launches", c.graph_launches},
+      {"library_pack_calls", c.library_pack_calls},
+      {"library_unpack_calls", c.library_unpack_calls},
+      {"commits_canonical", c.commits_canonical},
+      {"commits_unsupported", c.commits_unsupported},
+      {"model_inferences", c.model_inferences},
+      {"autotune_runs", c.autotune_runs},
+      {"verified_choices", c.verified_choices},
+      {"p2p_device", c.p2p_device},
+      {"p2p_oneshot", c.p2p_oneshot},
+      {"p2p_staged", c.p2p_staged},
+      {"p2p_library", c.p2p_library},
+      {"trial_launches", c.trial_launches},
+      {"method_model_choices", c.method_model_choices},
+      {"host_staged_calls", c.host_staged_calls},
+  };
+  for (int r = 0; r < size; ++r) {
+    if (r == rank) {
+      for (const auto &row : rows) {
+        TEMPI_LOG(LOG_DEBUG, "rank %d counters::%s: %lld", rank, row.first,
+                  (long long)row.second);
+      }
+    }
+    if (size > 1) {
+      MPI_Barrier(MPI_COMM_WORLD);
+    }
+  }
+}
 
 namespace {
 struct Runtime {
@@ -119,6 +173,7 @@ bool device_only(const void *p) {
 }
 
 void finalize_runtime() {
+  flush_release();
   Runtime &r = rt();
   std::lock_guard<std::mutex> lk(r.mu);
   if (r.ready.load(std::memory_order_acquire)) {

@@ -10,7 +10,10 @@
    with several objects per call, and one in ten of 64-256 MiB.
 2. Time every kernel variant on each with the autotuner's timer
    (tempi_time_variant: HIP events around back-to-back launches; copies of
-   64 MiB and more one launch at a time after a cache sweep).
+   64 MiB and more one launch at a time after a cache sweep), K rounds over
+   all variants (default 5), the label being each variant's median: a single
+   timing of a copy of a few KiB varies between runs by more than the
+   variants differ.
 3. Train the MLP (tempi_mlp_train: the library's HIP forward / backward /
    Adam kernels) to predict each variant's log2 time relative to the best
    variant, capped at 2x; features are normalized by the training set's
@@ -156,7 +159,10 @@ def generate(n, seed, max_packed):
     return cases
 
 
-def time_cases(cases, src, dst, log_every=2000, tag=""):
+def time_cases(cases, src, dst, log_every=2000, tag="", rounds=5):
+    """Features, per-variant times (us; the median of `rounds` timings, each
+    round timing every variant once), closed-form picks and packed bytes of
+    each copy."""
     L = ta.lib()
     nv = ta.num_variants()
     nf = L.tempi_mlp_num_features()
@@ -167,16 +173,19 @@ def time_cases(cases, src, dst, log_every=2000, tag=""):
     f = (ctypes.c_float * nf)()
     us = ctypes.c_float()
     t0 = time.time()
+    per = np.zeros((rounds, nv), np.float64)
     for i, (sb, base, direction, ppos, count) in enumerate(cases):
         strided, packed = src.addr(base), dst.addr(ppos)
         size = sb.size() * count
         reps = 10 if size < (1 << 20) else (4 if size < (8 << 20) else 2)
         ta.check(L.tempi_mlp_features(ctypes.byref(sb), count, strided, packed, direction, f))
         X[i] = np.frombuffer(f, np.float32)
-        for v in range(nv):
-            ta.check(L.tempi_time_variant(ctypes.byref(sb), count, strided, packed, direction, v,
-                                          0, reps, ctypes.byref(us)), "time_variant")
-            T[i, v] = max(float(us.value), 1e-3)
+        for k in range(rounds):
+            for v in range(nv):
+                ta.check(L.tempi_time_variant(ctypes.byref(sb), count, strided, packed, direction,
+                                              v, 0, reps, ctypes.byref(us)), "time_variant")
+                per[k, v] = max(float(us.value), 1e-3)
+        T[i] = np.median(per, axis=0)
         heur[i] = ta.select_variant(sb, count, strided, packed, direction, ta.SELECT_HEURISTIC)
         nbytes[i] = size
         if log_every and (i + 1) % log_every == 0:
@@ -204,6 +213,8 @@ def main(argv=None):
     ap.add_argument("--out", required=True)
     ap.add_argument("--report", required=True)
     ap.add_argument("--dataset", default=None, help="npz to save (or reuse) the timings")
+    ap.add_argument("--rounds", type=int, default=5,
+                    help="timings per variant and copy; the label is their median")
     args = ap.parse_args(argv)
 
     L = ta.lib()
@@ -219,13 +230,14 @@ def main(argv=None):
         dst = ta.DeviceBuffer(DST_BYTES + 4096)
         dst.fill(2)
         t0 = time.time()
-        tr = time_cases(generate(args.train, args.seed, args.max_packed), src, dst, tag="train")
+        tr = time_cases(generate(args.train, args.seed, args.max_packed), src, dst, tag="train",
+                        rounds=args.rounds)
         hcases = generate(args.heldout, args.heldout_seed, args.max_packed)
-        ho = time_cases(hcases, src, dst, tag="heldout")
+        ho = time_cases(hcases, src, dst, tag="heldout", rounds=args.rounds)
         # a second, independent timing of the held-out copies: every choice
         # (model, closed-form rule, and the autotuner's own pick from the
         # first timing) is judged on it
-        ho2 = time_cases(hcases, src, dst, tag="heldout-retime")
+        ho2 = time_cases(hcases, src, dst, tag="heldout-retime", rounds=args.rounds)
         data = {"Xtr": tr[0], "Ttr": tr[1], "Htr": tr[2], "Btr": tr[3],
                 "Xho": ho[0], "Tho": ho[1], "Hho": ho[2], "Bho": ho[3], "Tho2": ho2[1]}
         print(f"[selector_train] timing took {time.time() - t0:.0f} s", flush=True)
@@ -280,6 +292,7 @@ def main(argv=None):
     ta.check(L.tempi_mlp_save(args.out.encode(), params.ctypes.data, norm.ctypes.data), "save")
     report = {"train_samples": int(n), "heldout_samples": int(len(data["Xho"])),
               "variants": nv, "epochs": args.epochs, "train_seconds": round(train_s, 2),
+              "timings_per_label": args.rounds,
               "loss_first_last": [losses[0], losses[-1]]}
     for split, X, T, H, B in (("train", data["Xtr"], data["Ttr"], data["Htr"], data["Btr"]),
                               ("heldout", data["Xho"], data["Tho"], data["Hho"], data["Bho"])):
@@ -315,6 +328,20 @@ def main(argv=None):
                 "gbs_heuristic": float(B.sum() / (T2[idx, H].sum() * 1e-6) / 1e9),
                 "gbs_autotuner": float(B.sum() / (T2[idx, best].sum() * 1e-6) / 1e9),
             }
+            # copies of 256 KiB and more: where the choice moves bytes
+            mid = B >= (256 << 10)
+            if mid.any():
+                im = idx[mid]
+                report["heldout_retimed_ge_256KiB"] = {
+                    "copies": int(mid.sum()),
+                    "model_regret": regret_stats(T2[mid], pm[mid]),
+                    "autotuner_regret": regret_stats(T2[mid], best[mid]),
+                    "heuristic_regret": regret_stats(T2[mid], H[mid]),
+                    "model_regret_time_weighted":
+                        float(T2[im, pm[mid]].sum() / T2[mid].min(1).sum() - 1),
+                    "autotuner_regret_time_weighted":
+                        float(T2[im, best[mid]].sum() / T2[mid].min(1).sum() - 1),
+                }
             big = B >= BIG_MIN
             if big.any():
                 ib = idx[big]

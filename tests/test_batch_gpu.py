@@ -270,3 +270,44 @@ def test_batch_creation_leaves_buffers_alone():
     ref = np.full(n, 0xAB, np.uint8)
     orc.sb_unpack(desc, phost, ref, 0, 1)
     assert (dst.download() == ref).all()
+
+
+def test_one_graph_of_a_pack_and_an_unpack_batch(mpi):
+    """tempi_graph_create of a pack batch and the unpack batch that reads its
+    output: each replay packs the grid's faces as they are at that moment
+    and unpacks them into the second grid (eager uploads and plain launches
+    between the replays)."""
+    n, g = 40, 3
+    spec_faces = [tc.grid_face_subarray(n, g, f) for f in "xyz"]
+    blocks = []
+    for spec in spec_faces:
+        h = mpi.type_from_spec(spec)
+        blocks.append(mpi.strided_block(h))
+        mpi.free(h)
+    nbytes = (n + 2 * g) ** 3 * 4
+    grid, grid2 = ta.DeviceBuffer(nbytes), ta.DeviceBuffer(nbytes)
+    packed = [ta.DeviceBuffer(size) for _, size in blocks]
+    pb = ta.Batch([(sb, grid.addr(), p.addr(), 1, ta.DIR_PACK)
+                   for (sb, _), p in zip(blocks, packed)])
+    ub = ta.Batch([(sb, grid2.addr(), p.addr(), 1, ta.DIR_UNPACK)
+                   for (sb, _), p in zip(blocks, packed)])
+    graph = ta.Graph([pb, ub])
+    for rnd in range(4):
+        host = orc.splitmix_bytes(nbytes, 900 + rnd)
+        grid.upload(host)
+        grid2.fill(0xAB)
+        c0 = ta.counters()["graph_launches"]
+        graph.launch()
+        ta.synchronize()
+        assert ta.counters()["graph_launches"] == c0 + 1
+        ref = np.full(nbytes, 0xAB, np.uint8)
+        for (sb, size), p in zip(blocks, packed):
+            desc = orc.sb_desc(*sb.as_tuple())
+            exp = orc.sb_pack(desc, host, 0, 1)
+            assert (p.download() == exp).all(), rnd
+            orc.sb_unpack(desc, exp, ref, 0, 1)
+        assert (grid2.download() == ref).all(), rnd
+        # a plain launch of the pack batch between replays
+        pb.launch()
+        ta.synchronize()
+    graph.destroy()

@@ -503,3 +503,68 @@ def test_installed_method_model_is_consulted():
     m = L.tempi_p2p_method(1, 12345 * 1024, 1024, 3000)
     assert m in (ONESHOT, STAGED)
     assert ta.counters()["method_model_choices"] == c0 + 1
+
+
+STRUCT2 = ("struct", [3, 2], [0, 20], [("named", "MPI_FLOAT"), ("named", "MPI_DOUBLE")])
+
+
+def test_device_data_without_strided_form_sent_through_host_images(mpi):
+    """Isend/Irecv and blocking Send of a two-block struct on device buffers
+    (no strided form; the library cannot read device memory): the library
+    moves pinned host images of the buffers' spans, the receiver's bytes
+    between the blocks keep 0xAB, and the result equals MPICH's unpack of
+    MPICH's pack of the host copy."""
+    h = mpi.type_from_spec(STRUCT2)
+    lb, ext = mpi.extent(h)
+    count = 4
+    n = ext * count
+    host = orc.splitmix_bytes(n, 41)
+    size = mpi.pack_size(count, h)
+    packed = np.zeros(size, np.uint8)
+    assert mpi.pack(host.ctypes.data, count, h, packed.ctypes.data, size, library=True)[0] == 0
+    expected = np.full(n, 0xAB, np.uint8)
+    assert mpi.unpack(packed.ctypes.data, size, expected.ctypes.data, count, h,
+                      library=True)[0] == 0
+    src = ta.DeviceBuffer(n)
+    src.upload(host)
+    dst = ta.DeviceBuffer(n)
+    dst.fill(0xAB)
+    c0 = ta.counters()
+    err, rr = mpi.irecv(dst.addr(), count, h, 0, 31)
+    assert err == 0
+    err, sr = mpi.isend(src.addr(), count, h, 0, 31)
+    assert err == 0
+    err, sts, reqs = mpi.waitall([rr, sr], [h, h])
+    assert err == 0 and reqs == [mpi.REQUEST_NULL] * 2 and sts[0]["count"] == count
+    assert (dst.download() == expected).all()
+    assert ta.counters()["host_staged_calls"] == c0["host_staged_calls"] + 2
+    dst.fill(0xAB)
+    err, rr = mpi.irecv(dst.addr(), count, h, 0, 32)
+    assert err == 0
+    assert mpi.send(src.addr(), count, h, 0, 32) == 0
+    flag = 0
+    while not flag:
+        err, flag, st, rr = mpi.test(rr, h)
+        assert err == 0
+    assert (dst.download() == expected).all()
+    mpi.free(h)
+
+
+def test_contiguous_device_data_is_staged_without_gpu_aware_library(mpi):
+    """MPI_FLOAT device buffers (contiguous) are staged through the host
+    instead of reaching a library that cannot read device memory."""
+    h = mpi.build(("named", "MPI_FLOAT"))
+    host = orc.splitmix_bytes(4096, 43)
+    src = ta.DeviceBuffer(4096)
+    src.upload(host)
+    dst = ta.DeviceBuffer(4096)
+    dst.fill(0)
+    c0 = ta.counters()
+    err, rr = mpi.irecv(dst.addr(), 1024, h, 0, 33)
+    assert err == 0
+    err, sr = mpi.isend(src.addr(), 1024, h, 0, 33)
+    assert err == 0
+    err, _, _ = mpi.waitall([rr, sr], [h, h])
+    assert err == 0
+    assert (dst.download() == host).all()
+    assert ta.counters()["p2p_library"] == c0["p2p_library"]

@@ -497,3 +497,83 @@ def test_offsets_beyond_4gib():
     ta.synchronize()
     assert (dst.download(64, 0) == a).all() and (dst.download(64, stride) == b).all()
     assert (dst.download(64, 64) == 0xAB).all() and (dst.download(64, stride - 64) == 0xAB).all()
+
+
+NO_STRIDED_FORM = [
+    # two blocks of different element types
+    ("struct", [3, 2], [0, 20], [("named", "MPI_FLOAT"), ("named", "MPI_DOUBLE")]),
+    # irregular byte blocks
+    ("hindexed", [3, 5, 2, 9], [0, 7, 31, 40], tc.BYTE),
+    # a struct of a vector and a byte run at a negative displacement
+    ("struct", [1, 6], [8, -5], [("vector", 4, 2, 5, tc.BYTE), tc.BYTE]),
+]
+
+
+@pytest.mark.parametrize("spec", NO_STRIDED_FORM, ids=["struct2", "hindexed", "struct_neg"])
+def test_device_data_without_strided_form_is_staged_for_the_library(mpi, spec):
+    """A type with no strided form on device buffers goes to the MPI library
+    through pinned host images (the library here cannot read device memory;
+    the reference would hand it the device pointers): packed bytes equal
+    MPICH's on a host copy, unpack equals MPICH's unpack into 0xAB, and bytes
+    between the blocks keep their values."""
+    mpi.errors_return()
+    h = mpi.type_from_spec(spec)
+    lb, ext = mpi.extent(h)
+    incount = 3
+    origin = 64
+    length = origin + incount * ext + 64
+    host = orc.splitmix_bytes(length, 31)
+    size = mpi.pack_size(incount, h)
+    expected = np.zeros(size, np.uint8)
+    err, pos = mpi.pack(host.ctypes.data + origin, incount, h, expected.ctypes.data, size,
+                        library=True)
+    assert err == 0 and pos == size
+    src = ta.DeviceBuffer(length)
+    src.upload(host)
+    for packed_on_device in (True, False):
+        before = ta.counters()
+        if packed_on_device:
+            packed = ta.DeviceBuffer(size + 8)
+            packed.fill(0x5A)
+            err, pos = mpi.pack(src.addr(origin), incount, h, packed.addr(8), size)
+            got = packed.download()[8:]
+        else:
+            got = np.full(size, 0x5A, np.uint8)
+            err, pos = mpi.pack(src.addr(origin), incount, h, got.ctypes.data, size)
+        assert err == 0 and pos == size
+        assert (got == expected).all(), spec
+        assert ta.counters()["host_staged_calls"] == before["host_staged_calls"] + 1
+    ref = np.full(length, 0xAB, np.uint8)
+    err, pos = mpi.unpack(expected.ctypes.data, size, ref.ctypes.data + origin, incount, h,
+                          library=True)
+    assert err == 0 and pos == size
+    dst = ta.DeviceBuffer(length)
+    dst.fill(0xAB)
+    dpacked = ta.DeviceBuffer(size)
+    dpacked.upload(expected)
+    err, pos = mpi.unpack(dpacked.addr(), size, dst.addr(origin), incount, h)
+    assert err == 0 and pos == size
+    assert (dst.download() == ref).all(), spec
+    dst.fill(0xAB)
+    err, pos = mpi.unpack(expected.ctypes.data, size, dst.addr(origin), incount, h)
+    assert err == 0 and (dst.download() == ref).all()
+    err, _ = mpi.pack(src.addr(origin), incount, h, packed.addr(), size - 1)
+    assert err == mpi.ERR_TRUNCATE
+    mpi.free(h)
+
+
+def test_predefined_types_on_device_take_the_kernel_path(mpi):
+    """MPI_Pack of device data typed MPI_FLOAT (a predefined type, never
+    committed) runs the gfx950 copy instead of reaching the library."""
+    h = mpi.build(("named", "MPI_FLOAT"))
+    host = orc.splitmix_bytes(4000, 37)
+    src = ta.DeviceBuffer(4000)
+    src.upload(host)
+    out = ta.DeviceBuffer(4000)
+    before = ta.counters()
+    err, pos = mpi.pack(src.addr(), 1000, h, out.addr(), 4000)
+    assert err == 0 and pos == 4000
+    assert (out.download() == host).all()
+    after = ta.counters()
+    assert after["pack_kernel_launches"] == before["pack_kernel_launches"] + 1
+    assert after["library_pack_calls"] == before["library_pack_calls"]

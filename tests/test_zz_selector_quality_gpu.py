@@ -14,9 +14,10 @@ def test_installed_model_beats_the_closed_form_rule():
     """The shipped selector, on copies it was not trained on (a seed outside
     the training and held-out ranges), moves more packed bytes per second
     than the closed-form rule and about as many as the exhaustive autotuner.
-    Every variant is timed twice: the autotuner picks from the first timing,
-    and all three choices are judged on the second (the best of one noisy
-    timing, judged on that same timing, would be an unreachable bar)."""
+    Every variant is timed twice, each time the median of three timings: the
+    autotuner picks from the first timing, and all three choices are judged
+    on the second (the best of one noisy timing, judged on that same timing,
+    would be an unreachable bar)."""
     from tempi_amd import selector_train as st
 
     L = ta.lib()
@@ -30,8 +31,8 @@ def test_installed_model_beats_the_closed_form_rule():
     dst = ta.DeviceBuffer(st.DST_BYTES + 4096)
     dst.fill(2)
     print(f"timing {len(cases)} copies x {ta.num_variants()} variants, twice", flush=True)
-    _, T1, heur, nbytes = st.time_cases(cases, src, dst, log_every=0)
-    _, T2, _, _ = st.time_cases(cases, src, dst, log_every=0)
+    _, T1, heur, nbytes = st.time_cases(cases, src, dst, log_every=0, rounds=3)
+    _, T2, _, _ = st.time_cases(cases, src, dst, log_every=0, rounds=3)
     model = np.array([ta.select_variant(sb, count, src.addr(base), dst.addr(ppos), d,
                                         ta.SELECT_MODEL)
                       for sb, base, d, ppos, count in cases])
@@ -43,4 +44,4 @@ def test_installed_model_beats_the_closed_form_rule():
     }
     print(gbs, flush=True)
     assert gbs["model"] >= 1.05 * gbs["rule"], gbs
-    assert gbs["model"] >= 0.9 * gbs["autotuner"], gbs
+    assert gbs["model"] >= 0.95 * gbs["autotuner"], gbs

@@ -220,3 +220,35 @@ def sweep_type(rng: random.Random, max_packed: int = 256 << 20, min_packed: int 
             packed = rows * bl * planes
         if min_packed <= packed <= max_packed:
             return spec, rng.randint(0, 15), packed
+
+
+def random_requests(seed: int = 4, nreq: int = 4096):
+    """The batched random-request workload: `nreq` 1D/2D/3D strided blocks
+    with packed sizes log-uniform in [64 B, 16 MiB], starts 0-12 B, all
+    reading one source buffer.  Returns ([(start, extent, counts, strides)],
+    bytes the source buffer needs)."""
+    import math
+
+    rng = random.Random(seed)
+    out = []
+    span_max = 0
+    for _ in range(nreq):
+        target = int(math.exp(rng.uniform(math.log(64), math.log(16 << 20))))
+        nd = rng.choice([1, 2, 3])
+        if nd == 1:
+            counts, strides = [target], [1]
+        else:
+            bl = rng.choice([4, 8, 12, 16, 32, 64, 128, 256, 512, 1024, 2048])
+            bl = min(bl, target)
+            rows = max(1, target // bl)
+            if nd == 2:
+                counts, strides = [bl, rows], [1, bl + rng.randint(0, 512)]
+            else:
+                r1 = max(1, int(math.sqrt(rows)))
+                r2 = max(1, rows // r1)
+                s1 = bl + rng.randint(0, 512)
+                counts, strides = [bl, r1, r2], [1, s1, s1 * r1 + rng.randint(0, 4096)]
+        span = 1 + sum((c - 1) * s for c, s in zip(counts, strides)) + counts[0]
+        out.append((rng.randint(0, 3) * 4, span + 16, counts, strides))
+        span_max = max(span_max, span + 64)
+    return out, span_max
