@@ -965,15 +965,18 @@ def bench_reference_sweep():
                      "2d_byte_subarray": "subarray"}[r["layout"]]
             row[f"{mode}_{short}_pack_us"] = round(r["pack_s"] * 1e6, 2)
             row[f"{mode}_{short}_unpack_us"] = round(r["unpack_s"] * 1e6, 2)
-            row[f"{mode}_{short}_pack_mibs"] = round(r["bytes"] / r["pack_s"] / 2 ** 20, 1)
+            row[f"{mode}_{short}_pack_mibs"] = (round(r["bytes"] / r["pack_s"] / 2 ** 20, 1)
+                                                if r["pack_s"] > 0 else None)
     table = sorted(rows.values(), key=lambda r: (r["target"], r["count"], r["block"]))
     summary = {}
     for target in (1024, 1 << 20, 4 << 20):
         sel = [r for r in table if r["target"] == target]
         sp = [r[f"device_{k}_pack_mibs"] / r[f"mpi_library_host_{k}_pack_mibs"]
-              for r in sel for k in ("vector", "hvector", "subarray")]
+              for r in sel for k in ("vector", "hvector", "subarray")
+              if r.get(f"device_{k}_pack_mibs") and r.get(f"mpi_library_host_{k}_pack_mibs")]
         dev = [r["bytes"] / (r[f"device_{k}_pack_us"] * 1e-6) / 1e9
-               for r in sel for k in ("vector", "hvector", "subarray")]
+               for r in sel for k in ("vector", "hvector", "subarray")
+               if r.get(f"device_{k}_pack_us")]
         summary[f"packed_{target}B"] = {
             "device_pack_gbs_median": round(float(np.median(dev)), 2),
             "speedup_vs_mpi_library_geomean": round(float(np.exp(np.mean(np.log(sp)))), 1),

@@ -18,11 +18,14 @@
 //
 // One JSON line per case; times are the trimean (Q1 + 2 median + Q3) / 4 of
 // the iterations, the reference's statistic (src/internal/statistics.cpp:
-// 30-39), in seconds.
+// 30-39), in seconds, from std::chrono::steady_clock (MPI_Wtime, which the
+// reference reads, has microsecond ticks in some MPI builds: too coarse for
+// the smallest host packs).
 #include <hip/hip_runtime.h>
 #include <mpi.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -36,6 +39,12 @@
       exit(2);                                                                 \
     }                                                                          \
   } while (0)
+
+static double now() {
+  return std::chrono::duration<double>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
 
 static double trimean(std::vector<double> v) {
   std::sort(v.begin(), v.end());
@@ -104,12 +113,12 @@ static int run_pack(const std::string &mode, int iters) {
           std::vector<double> tp, tu;
           for (int it = -untimed; it < iters; ++it) {
             int pos = 0;
-            double t0 = MPI_Wtime();
+            double t0 = now();
             MPI_Pack(src, count, t, dst, packed, &pos, MPI_COMM_WORLD);
-            double t1 = MPI_Wtime();
+            double t1 = now();
             pos = 0;
             MPI_Unpack(dst, packed, &pos, src, count, t, MPI_COMM_WORLD);
-            double t2 = MPI_Wtime();
+            double t2 = now();
             if (it >= 0) {
               tp.push_back(t1 - t0);
               tu.push_back(t2 - t1);
@@ -196,11 +205,11 @@ static int run_commit(int iters) {
     for (int kind = 0; kind < 5; ++kind) {
       std::vector<double> tc, tm;
       for (int it = -2; it < iters; ++it) {
-        double t0 = MPI_Wtime();
+        double t0 = now();
         MPI_Datatype t = make_3d(kind, d, alloc);
-        double t1 = MPI_Wtime();
+        double t1 = now();
         MPI_Type_commit(&t);
-        double t2 = MPI_Wtime();
+        double t2 = now();
         MPI_Type_free(&t);
         if (it >= 0) {
           tm.push_back(t1 - t0);
