@@ -300,6 +300,23 @@ int tempi_batch_info(const tempi_batch *batch, int64_t *requests,
 int tempi_batch_variant(const tempi_batch *batch, int64_t request,
                         int *variant);
 int tempi_batch_destroy(tempi_batch *batch);
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): resolves `reqs` and launches them as one batch on
+   `stream` (NULL: the library stream) like tempi_batch_create +
+   tempi_batch_launch, but the batch's request and tile tables go to a
+   static device array of 4 MiB that the library holds on each GPU until the
+   process ends, not to memory it allocates; nothing is allocated on the
+   device and nothing waits for the launch.  Requests with variant -1 take
+   the closed-form rule (TEMPI_SELECT_HEURISTIC: the learned selector and
+   trial timings would allocate).  Batches whose tables exceed the array
+   (about 4 000 requests, or 2^19 tiles) fail with TEMPI_ERR_UNSUPPORTED
+   before any device work.  Every call on a GPU overwrites that GPU's
+   array: calls on one stream follow each other in stream order, but calls
+   that overlap on one GPU (other streams, other threads without ordering)
+   race on the tables. */
+int tempi_batch_run_noalloc(const tempi_batch_request *reqs, int64_t n,
+                            void *stream);
 
 /* Learned variant selector: replaces the reference's measured tables and
    interpolation (src/internal/measure_system.cpp:181-299) and its fixed

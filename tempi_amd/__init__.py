@@ -216,6 +216,7 @@ def _prototypes(L):
         "tempi_batch_info": (_I, [_P, i64_p, i64_p, i64_p]),
         "tempi_batch_variant": (_I, [_P, _I64, ctypes.POINTER(_I)]),
         "tempi_batch_destroy": (_I, [_P]),
+        "tempi_batch_run_noalloc": (_I, [ctypes.POINTER(BatchRequest), _I64, _P]),
         "tempi_graph_create": (_I, [ctypes.POINTER(_P), _I64, ctypes.POINTER(_P)]),
         "tempi_graph_launch": (_I, [_P, _P]),
         "tempi_graph_destroy": (_I, [_P]),
@@ -473,6 +474,14 @@ def batch_requests(requests):
         arr[i].direction = direction
         arr[i].variant = r[5] if len(r) > 5 else -1
     return arr, len(reqs)
+
+
+def batch_run_noalloc(requests, stream_=None):
+    """tempi_batch_run_noalloc: resolve and launch the requests as one batch
+    whose tables sit in the library's static device array (no allocation);
+    asynchronous on the stream."""
+    arr, n = batch_requests(requests)
+    check(lib().tempi_batch_run_noalloc(arr, n, stream_), "batch_run_noalloc")
 
 
 class Batch:

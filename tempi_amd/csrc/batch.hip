@@ -668,3 +668,127 @@ TEMPI_EXPORT int tempi_batch_destroy(tempi_batch *b) {
   delete b;
   return TEMPI_SUCCESS;
 }
+
+// ---------------------------------------------------------------------------
+// the batch entry point that allocates nothing
+// ---------------------------------------------------------------------------
+//
+// tempi_batch_run_noalloc resolves and launches a batch in one call, with
+// the request and tile tables in a file-scope device array instead of
+// memory the library allocates, for callers that cannot allocate device
+// memory.  The array's host-side name is only a stand-in for it: host code
+// uses the address hipGetSymbolAddress reports for the current device,
+// taken once per device on the first call, and never the name itself.
+
+namespace tempi {
+namespace {
+
+// largest table image (request parameters, tile records, XCD bounds) a
+// call can place: 4 MiB holds about 4 000 requests over 2^19 tiles (the
+// 1 664-request halo batch of 64 subdomains of 128^3 needs 0.5 MiB)
+constexpr size_t kScratchBytes = 4u << 20;
+
+struct ScratchSlot {
+  std::once_flag once;
+  int err = TEMPI_SUCCESS;
+  char *ptr = nullptr;             // the array's address on this device
+  hipEvent_t uploaded = nullptr;   // the last call's table upload
+  std::mutex mu;
+  std::vector<char> image;         // that upload's host source
+};
+constexpr int kMaxDevices = 64;
+ScratchSlot g_scratch_slots[kMaxDevices];
+
+} // namespace
+} // namespace tempi
+
+static __device__ __align__(256) char scratch[tempi::kScratchBytes];
+
+namespace tempi {
+namespace {
+
+int scratch_slot(ScratchSlot **out) {
+  int dev = 0;
+  TEMPI_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    return set_error(TEMPI_ERR_UNSUPPORTED, "device %d", dev);
+  }
+  ScratchSlot &sl = g_scratch_slots[dev];
+  std::call_once(sl.once, [&sl] {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      sl.err = set_error(TEMPI_ERR_HIP, "hipGetSymbolAddress(scratch): %s",
+                         hipGetErrorString(e));
+      return;
+    }
+    e = hipEventCreateWithFlags(&sl.uploaded, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      sl.err = set_error(TEMPI_ERR_HIP, "event creation: %s",
+                         hipGetErrorString(e));
+      return;
+    }
+    sl.ptr = static_cast<char *>(p);
+  });
+  if (sl.err != TEMPI_SUCCESS) {
+    return set_error(sl.err, "static scratch of device %d is unusable", dev);
+  }
+  *out = &sl;
+  return TEMPI_SUCCESS;
+}
+
+} // namespace
+} // namespace tempi
+
+TEMPI_EXPORT int tempi_batch_run_noalloc(const tempi_batch_request *reqs,
+                                         int64_t n, void *stream) {
+  if ((n > 0 && !reqs) || n < 0) {
+    return set_error(TEMPI_ERR_ARG, "bad argument");
+  }
+  std::vector<Req> rs;
+  std::vector<int64_t> where;
+  int64_t bytes = 0;
+  // the closed-form rule: the learned selector and trial timings would
+  // allocate device memory on first use
+  int err = resolve_requests(
+      reqs, n,
+      [](const Req &q) { return heuristic_variant(q.g, q.ds, q.dp, q.dir); },
+      &rs, &where, &bytes);
+  if (err) {
+    return err;
+  }
+  if (rs.empty()) {
+    return TEMPI_SUCCESS;
+  }
+  BatchPlan pl;
+  if ((err = plan_batch(rs, device_cu_count(), &pl))) {
+    return err;
+  }
+  if (pl.image.size() > kScratchBytes) {
+    return set_error(TEMPI_ERR_UNSUPPORTED,
+                     "batch tables of %zu B (%zu requests, %u tiles) exceed "
+                     "the %zu B of static scratch",
+                     pl.image.size(), rs.size(), pl.launch.ntiles,
+                     kScratchBytes);
+  }
+  ScratchSlot *sl = nullptr;
+  if ((err = scratch_slot(&sl))) {
+    return err;
+  }
+  TraceRange trace("tempi_batch_run_noalloc");
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : library_stream();
+  std::lock_guard<std::mutex> lk(sl->mu);
+  // the previous upload has read its host image (a copy from pageable
+  // memory may finish after the call that enqueued it returned)
+  TEMPI_HIP_CHECK(hipEventSynchronize(sl->uploaded));
+  sl->image.swap(pl.image);
+  TEMPI_HIP_CHECK(hipMemcpyAsync(sl->ptr, sl->image.data(), sl->image.size(),
+                                 hipMemcpyHostToDevice, s));
+  TEMPI_HIP_CHECK(hipEventRecord(sl->uploaded, s));
+  launch_kernel(placed(pl, sl->ptr), s);
+  TEMPI_HIP_CHECK(hipGetLastError());
+  ++counters().batch_launches;
+  return TEMPI_SUCCESS;
+}
