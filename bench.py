@@ -79,6 +79,9 @@ def main():
                     help="only the 256 MiB 2D packs/unpacks (64 B - 4 KiB blocks, aligned "
                          "and 4 B off), three cold launches each, then the 256 MiB "
                          "contiguous calibration copy (for rocprofv3)")
+    ap.add_argument("--profile-sweep", action="store_true",
+                    help="the sweep's copies of >= 64-B blocks and >= 16 MiB alone, each pack "
+                         "and unpack launched 3 times after a cache sweep (for rocprofv3)")
     ap.add_argument("--profile-halo", action="store_true",
                     help="only the halo batch (1664 requests), warm and cold launches, then "
                          "the 256 MiB contiguous calibration copy (for rocprofv3)")
@@ -107,6 +110,9 @@ def main():
         return
     if args.profile_halo:
         profile_halo(ta, m, tc, stream)
+        return
+    if args.profile_sweep:
+        profile_sweep(ta, m, tc, stream)
         return
     n, g = 512, 3
     full = n + 2 * g
@@ -191,7 +197,19 @@ def main():
     wall_g, _ = timed_loop(ta, lambda: (pack_batch.graph_launch(stream),
                                         unpack_batch.graph_launch(stream)), args.steps, stream)
     extras["batched_graph"] = {"us_per_step": round(wall_g * 1e6 / args.steps, 3),
-                               "gbs": round(packed_per_step * args.steps / wall_g / 1e9, 1)}
+                               "gbs": round(packed_per_step * args.steps / wall_g / 1e9, 1),
+                               "form": "two graph launches per step, one per batch"}
+    # the whole step as one graph: the pack batch's launch, then the unpack
+    # batch's (tempi_graph_create)
+    step_graph = ta.Graph([pack_batch, unpack_batch])
+    for _ in range(args.warmup):
+        step_graph.launch(stream)
+    wall_sg, _ = timed_loop(ta, lambda: step_graph.launch(stream), args.steps, stream)
+    extras["step_graph"] = {"us_per_step": round(wall_sg * 1e6 / args.steps, 3),
+                            "gbs": round(packed_per_step * args.steps / wall_sg / 1e9, 1),
+                            "form": "one graph of both batch launches per step"}
+    step_graph.destroy()
+    log(f"graph replay: {extras['batched_graph']}, whole step: {extras['step_graph']}")
 
     # per-kernel device time, each single-copy kernel launched back to back
     kernels = []
@@ -491,6 +509,58 @@ def bench_large_blocks(ta, stream):
     return res
 
 
+def profile_sweep(ta, m, tc, stream, ncases=48, reps=3):
+    """The sweep cases of >= 64-B blocks and >= 16 MiB packed (seeds 1000 + k,
+    as bench_sweep), each packed and then unpacked with the library's
+    default choice, `reps` launches each after a read-only 1 GiB sweep (the
+    trace: k_touch, then the copy kernel, per launch), then one 256 MiB
+    contiguous copy (variant 1, the FETCH_SIZE / WRITE_SIZE calibration).
+    One JSON line per launch, in trace order, with the case's line bound."""
+    for k in range(ncases):
+        spec, base, packed_bytes = tc.sweep_type(random.Random(1000 + k))
+        if packed_bytes < (16 << 20):
+            continue
+        h = m.type_from_spec(spec)
+        sb, size = m.strided_block(h)
+        m.free(h)
+        if sb.counts[0] < 64 or base + sb.extent > (8 << 30):
+            continue
+        src = ta.DeviceBuffer(base + sb.extent + 64)
+        dst = ta.DeviceBuffer(size + 64)
+        ls, lp = line_bound(sb, src.addr(base) % 128)
+        ev = [ta.Event(), ta.Event()]
+        for d, name in ((ta.DIR_PACK, "pack"), (ta.DIR_UNPACK, "unpack")):
+            def copy(variant, d=d):
+                if d == ta.DIR_PACK:
+                    ta.pack_sb(sb, src.addr(base), 1, dst.addr(), size, 0, variant, stream)
+                else:
+                    ta.unpack_sb(sb, dst.addr(), size, src.addr(base), 1, 0, variant, stream)
+
+            # the first call settles the variant (its trials precede the
+            # traced launches)
+            copy(-1)
+            ta.synchronize(stream)
+            v = ta.select_variant(sb, 1, src.addr(base), dst.addr(), d, ta.SELECT_DEFAULT)
+            for _ in range(reps):
+                ta.flush_caches()
+                ev[0].record(stream)
+                copy(v)
+                ev[1].record(stream)
+                ta.synchronize(stream)
+                print(json.dumps({"case": k, "op": name, "variant": v, "block": int(sb.counts[0]),
+                                  "packed_bytes": size, "strided_lines": ls, "packed_lines": lp,
+                                  "line_bound_frac": round(2 * size / (128 * (ls + lp)), 4),
+                                  "us": round(ev[0].elapsed_ms(ev[1]) * 1e3, 2)}), flush=True)
+        src.free()
+        dst.free()
+    cal = 256 << 20
+    a, c = ta.DeviceBuffer(cal), ta.DeviceBuffer(cal)
+    a.fill(1)
+    ta.pack_sb(ta.StridedBlock.make(0, cal, [cal], [1]), a.addr(), 1, c.addr(), cal, 0, 1,
+               stream)
+    ta.synchronize(stream)
+
+
 def halo_requests(ta, m, tc, hosts=None):
     """The pack requests of 26 halo subarrays (radius 3, float64) of 64
     subdomains of 128^3, into one packed buffer: (requests, total packed
@@ -578,6 +648,15 @@ def bench_halos(ta, m, tc, orc, stream):
         off += exp.size
     del got
     wall_g, _ = timed_loop(ta, lambda: b.graph_launch(stream), reps, stream)
+    # the same requests with the closed-form rule's variants
+    br = ta.Batch(reqs, method=ta.SELECT_HEURISTIC)
+    for _ in range(5):
+        br.launch(stream)
+    _, dev_r = timed_loop(ta, lambda: br.launch(stream), reps, stream)
+    picks = {}
+    for i in range(26):
+        picks[str(i)] = [b.variant(i), br.variant(i)]
+    br.destroy()
 
     def per_call():
         o = 0
@@ -608,6 +687,8 @@ def bench_halos(ta, m, tc, orc, stream):
         "per_call_async_packs_per_s": round(nreq / wall_pa, 1),
         "batched_us_per_launch": round(wall_b * 1e6 / reps, 2),
         "batch_kernel_us": round(dev_b * 1e6 / reps, 2),
+        "batch_kernel_us_rule_variants": round(dev_r * 1e6 / reps, 2),
+        "variants_default_vs_rule_first_subdomain": picks,
         "batch_kernel_hbm_gbs": round(2 * total / (dev_b / reps) / 1e9, 1),
         "bit_exact": exact,
     }
@@ -754,6 +835,27 @@ def bench_vector(ta, m, tc, stream):
             "packed_bytes": size}
 
 
+def line_bound(sb, strided_mod, count=1):
+    """Distinct 128-B lines a copy touches: (strided-side lines, packed-side
+    lines).  `strided_mod` is the strided origin's address modulo 128 (the
+    packed buffer starts a line).  Rows are visited in address order (the
+    sweep's strides are positive and each dimension's stride exceeds the
+    span inside it), so a line shared by two rows is shared by neighbours."""
+    import numpy as np
+
+    counts = [int(sb.counts[i]) for i in range(sb.ndims)]
+    strides = [int(sb.strides[i]) for i in range(sb.ndims)]
+    L = counts[0]
+    starts = np.zeros(1, np.int64)
+    for c, st in list(zip(counts[1:], strides[1:])) + [(count, int(sb.extent))]:
+        starts = ((np.arange(c, dtype=np.int64) * st)[:, None] + starts[None, :]).reshape(-1)
+    a = starts + int(sb.start) + strided_mod
+    first, last = a // 128, (a + L - 1) // 128
+    lines = int((last - first + 1).sum()) - int((first[1:] == last[:-1]).sum())
+    packed = L * len(starts)
+    return lines, (packed + 127) // 128
+
+
 def bench_sweep(ta, m, tc, stream, ncases=48, max_span=8 << 30):
     """The 2D vector / 3D hvector-of-vector byte sweep (block lengths 1 B -
     1 KiB, stride = block + U{0..4096}, base offset U{0..15}, 1 KiB - 256 MiB
@@ -779,6 +881,11 @@ def bench_sweep(ta, m, tc, stream, ncases=48, max_span=8 << 30):
         src = ta.DeviceBuffer(span + 64)
         dst = ta.DeviceBuffer(size + 64)
         row = {"block": int(sb.counts[0]) if sb.ndims else size, "packed_bytes": size}
+        # the line-granular bound: 128-B lines touched on both sides, each
+        # moved once at the HBM peak
+        ls, lp = line_bound(sb, src.addr(base) % 128)
+        row["lines"] = ls + lp
+        row["line_bound_frac"] = round(2 * size / (128 * (ls + lp)), 4)
         for d, name in ((ta.DIR_PACK, "pack"), (ta.DIR_UNPACK, "unpack")):
             # one call settles the default choice, which is then timed the way
             # the library times copies (cold from 64 MiB up)
@@ -791,6 +898,7 @@ def bench_sweep(ta, m, tc, stream, ncases=48, max_span=8 << 30):
             us = ta.time_variant(sb, 1, src.addr(base), dst.addr(), d, v, 0, 3)
             row[f"{name}_us"] = round(us, 2)
             row[f"{name}_frac"] = round(2 * size / us / 1e3 / HBM_PEAK_GBS, 4)
+            row[f"{name}_of_line_bound"] = round(row[f"{name}_frac"] / row["line_bound_frac"], 4)
             row[f"{name}_variant"] = v
         src.free()
         dst.free()
@@ -803,9 +911,14 @@ def bench_sweep(ta, m, tc, stream, ncases=48, max_span=8 << 30):
         big = [c for c in cases if c["block"] >= 64 and c["packed_bytes"] >= (16 << 20)]
         if big:
             fr = [c[f"{name}_frac"] for c in big]
+            lb = [c[f"{name}_of_line_bound"] for c in big]
             res[f"{name}_hbm_frac_block_ge64_ge16MiB"] = {
                 "n": len(big), "median": round(float(np.median(fr)), 4),
-                "min": round(float(np.min(fr)), 4)}
+                "min": round(float(np.min(fr)), 4),
+                "line_bound_median": round(float(np.median(
+                    [c["line_bound_frac"] for c in big])), 4),
+                "of_line_bound_median": round(float(np.median(lb)), 4),
+                "of_line_bound_min": round(float(np.min(lb)), 4)}
     res["per_case"] = cases
     log(f"vector sweep: { {k: v for k, v in res.items() if k != 'per_case'} }")
     return res
@@ -979,8 +1092,8 @@ def bench_reference_sweep():
                if r.get(f"device_{k}_pack_us")]
         summary[f"packed_{target}B"] = {
             "device_pack_gbs_median": round(float(np.median(dev)), 2),
-            "speedup_vs_mpi_library_geomean": round(float(np.exp(np.mean(np.log(sp)))), 1),
-            "speedup_vs_mpi_library_min": round(float(np.min(sp)), 2),
+            "speedup_vs_mpi_library_geomean": round(float(np.exp(np.mean(np.log(sp)))), 3),
+            "speedup_vs_mpi_library_min": round(float(np.min(sp)), 3),
             "speedup_vs_mpi_library_max": round(float(np.max(sp)), 1)}
     res = {"summary": summary, "rows": table,
            "note": "wall time of one synchronous call from a native MPI program; device and "
@@ -1020,13 +1133,14 @@ def bench_type_commit():
 
 def bench_first_call(ta, m, tc):
     """Wall time of the first synchronous MPI_Pack / MPI_Unpack of a
-    256 MiB shape (64-B rows, 2 x 64 + 64 B apart) against later calls:
+    256 MiB shape (64-B rows, 2 x 64 + 80 B apart) against later calls:
     under the default policy the first call of a copy of 64 MiB or more
     times the model's candidates once, cold."""
     import numpy as np
 
+    # a stride no other copy of the run uses: the shape is new to the library
     rows, bl = 1 << 22, 64
-    spec = ("vector", rows, bl, 2 * bl + 64, tc.BYTE)
+    spec = ("vector", rows, bl, 2 * bl + 80, tc.BYTE)
     h = m.type_from_spec(spec)
     sb, size = m.strided_block(h)
     src = ta.DeviceBuffer(sb.extent + 64)

@@ -31,10 +31,11 @@ namespace tempi {
 namespace {
 
 struct BatchItem {
-  KParams p;
   int32_t kind;            // 4 * variant + 2 * aligned + direction
   uint32_t units_per_tile; // units of one block-pass: groups per block * K
   uint32_t ntiles;         // tiles of this request
+  uint32_t pad;
+  KParams p;               // after the kind: both in the item's first lines
 };
 
 constexpr uint32_t kXcds = 8;
@@ -146,9 +147,13 @@ struct SmallIndex {
   uint32_t ntiles[kSmallItems];   // tiles of each request
 };
 
-// The index is read in place through the kernel-argument segment pointer (it
-// is the first argument, at offset 0): indexing the by-value parameter with
-// a run-time request number would copy it to scratch memory.
+// The index is read through the kernel-argument segment pointer (it is the
+// first argument, at offset 0) into registers, every entry unconditionally
+// and before the loop: the entries past the last request hold `end`, so no
+// test of the request count is needed, and the loads issue together.  (With
+// the count tested per entry, or with entries read at a run-time index, the
+// compiler chained one scalar load and wait per request in front of the
+// first data load, or copied the parameter to scratch memory.)
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(5)))
     k_batch_small(const SmallIndex arg, const BatchItem *__restrict__ items) {
@@ -156,17 +161,29 @@ __global__ void __launch_bounds__(kBlock)
   const SmallIndex *ix =
       (const SmallIndex *)__builtin_amdgcn_kernarg_segment_ptr();
   const uint32_t end = ix->end;
+  uint32_t base[kSmallItems], share[kSmallItems], ntiles[kSmallItems];
+#pragma unroll
+  for (int i = 0; i < kSmallItems; ++i) {
+    base[i] = ix->base[i];
+    share[i] = ix->share[i];
+    ntiles[i] = ix->ntiles[i];
+  }
   for (uint32_t t = blockIdx.x; t < end; t += gridDim.x) {
-    uint32_t r = 0;
+    // request r: the last whose range starts at or before t
+    uint32_t r = 0, b = base[0], sh = share[0], nt = ntiles[0];
 #pragma unroll
     for (int i = 1; i < kSmallItems; ++i) {
-      r += (i < (int)ix->n && t >= ix->base[i]) ? 1u : 0u;
+      const bool in = t >= base[i];
+      r = in ? (uint32_t)i : r;
+      b = in ? base[i] : b;
+      sh = in ? share[i] : sh;
+      nt = in ? ntiles[i] : nt;
     }
-    const uint32_t q = t - ix->base[r];
-    const uint32_t l = (q % kXcds) * ix->share[r] + q / kXcds;
-    if (l < ix->ntiles[r]) {
+    const uint32_t q = t - b;
+    const uint32_t l = (q % kXcds) * sh + q / kXcds;
+    if (l < nt) {
       const BatchItem &it = items[r];
-      run_tile(it.kind, it.p, l, it.ntiles);
+      run_tile(it.kind, it.p, l, nt);
     }
   }
 }
