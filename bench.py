@@ -177,6 +177,24 @@ def main():
         "unpack": [unpack_batch.variant(i) for i in range(len(faces))],
     }
     log(f"batch variants (x, y, z faces): {extras['batch_variants']}")
+    # the same two batches with the closed-form rule's variants
+    rb = [ta.Batch([(f["sb"], grid.addr(), f["packed"].addr(), 1, ta.DIR_PACK)
+                    for f in faces], method=ta.SELECT_HEURISTIC),
+          ta.Batch([(f["sb"], grid2.addr(), f["packed"].addr(), 1, ta.DIR_UNPACK)
+                    for f in faces], method=ta.SELECT_HEURISTIC)]
+    for _ in range(args.warmup):
+        rb[0].launch(stream)
+        rb[1].launch(stream)
+    wall_r, dev_r = timed_loop(ta, lambda: (rb[0].launch(stream), rb[1].launch(stream)),
+                               args.steps, stream)
+    extras["batch_rule_variants"] = {
+        "pack": [rb[0].variant(i) for i in range(len(faces))],
+        "unpack": [rb[1].variant(i) for i in range(len(faces))],
+        "us_per_step": round(wall_r * 1e6 / args.steps, 3),
+        "kernel_us_per_launch": round(dev_r * 1e6 / (2 * args.steps), 3)}
+    for b in rb:
+        b.destroy()
+    log(f"rule variants: {extras['batch_rule_variants']}")
     # -- the same step as six single-copy kernels (what MPI_Pack launches)
     def step_per_call():
         for f in faces:
@@ -513,9 +531,11 @@ def profile_sweep(ta, m, tc, stream, ncases=48, reps=3):
     """The sweep cases of >= 64-B blocks and >= 16 MiB packed (seeds 1000 + k,
     as bench_sweep), each packed and then unpacked with the library's
     default choice, `reps` launches each after a read-only 1 GiB sweep (the
-    trace: k_touch, then the copy kernel, per launch), then one 256 MiB
-    contiguous copy (variant 1, the FETCH_SIZE / WRITE_SIZE calibration).
-    One JSON line per launch, in trace order, with the case's line bound."""
+    trace: k_empty marks each case and direction, then the settling call's
+    kernels, then k_touch and the copy kernel per timed launch), then one
+    256 MiB contiguous copy (variant 1, the FETCH_SIZE / WRITE_SIZE
+    calibration).  One JSON line per timed launch, in trace order, with the
+    case's line bound."""
     for k in range(ncases):
         spec, base, packed_bytes = tc.sweep_type(random.Random(1000 + k))
         if packed_bytes < (16 << 20):
@@ -536,8 +556,10 @@ def profile_sweep(ta, m, tc, stream, ncases=48, reps=3):
                 else:
                     ta.unpack_sb(sb, dst.addr(), size, src.addr(base), 1, 0, variant, stream)
 
-            # the first call settles the variant (its trials precede the
-            # traced launches)
+            # k_empty marks the start of each case and direction in the
+            # trace; the first call settles the variant (its trials precede
+            # the timed launches, which are the group's last `reps` copies)
+            ta.launch_latency_us(1)
             copy(-1)
             ta.synchronize(stream)
             v = ta.select_variant(sb, 1, src.addr(base), dst.addr(), d, ta.SELECT_DEFAULT)
@@ -771,6 +793,12 @@ def bench_random_requests(ta, tc, orc, stream, nreq=4096, seed=4):
     b.launch(stream)
     reps = 5
     wall_b, dev_b = timed_loop(ta, lambda: b.launch(stream), reps, stream)
+    br = ta.Batch(reqs, method=ta.SELECT_HEURISTIC)
+    br.launch(stream)
+    _, dev_r = timed_loop(ta, lambda: br.launch(stream), reps, stream)
+    br.destroy()
+    b.launch(stream)
+    ta.synchronize(stream)
     got = packed.download()
     exact, off = True, 0
     for sb in reqs_geo:
@@ -794,6 +822,8 @@ def bench_random_requests(ta, tc, orc, stream, nreq=4096, seed=4):
            "batched_packs_per_s": round(nreq * reps / wall_b, 1),
            "per_call_sync_packs_per_s": round(nreq / wall_pc, 1),
            "batch_kernel_hbm_gbs": round(2 * total / (dev_b / reps) / 1e9, 1),
+           "batch_kernel_us": round(dev_b * 1e6 / reps, 2),
+           "batch_kernel_us_rule_variants": round(dev_r * 1e6 / reps, 2),
            "bit_exact": exact}
     res["speedup"] = round(res["batched_packs_per_s"] / res["per_call_sync_packs_per_s"], 1)
     log(f"random requests: {res}")

@@ -271,6 +271,8 @@ bool model_available();
 int model_variant(const Geom &g, const char *strided, const char *packed,
                   int dir, const std::string &key); // -1: no model
 void model_precommit(const tempi_strided_block &sb);
+// waits for commit-time inferences still in flight (at MPI_Finalize)
+void model_finalize();
 // inputs of the selector networks (variants and staging methods)
 constexpr int kModelInputs = 16;
 // the staging-method model (mlp.hip): features of a message of `bytes`

@@ -312,6 +312,7 @@ TEMPI_EXPORT int MPI_Finalize(void) {
     }
     neighbor_finalize();
     p2p_finalize();
+    model_finalize();
     finalize_runtime();
   }
   return real_mpi().Finalize();

@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <dlfcn.h>
+#include <deque>
 #include <map>
 #include <random>
 #include <string>
@@ -297,6 +298,13 @@ __global__ void __launch_bounds__(kThreads)
 // host side
 // ---------------------------------------------------------------------------
 
+// Commit-time inferences run without waiting (model_precommit): each takes
+// a slot of pinned host memory that the kernel reads and writes through its
+// device mapping -- the type's two feature vectors and two picks -- until
+// its event has fired and its picks are in the shape cache.
+constexpr int kSlots = 16;
+constexpr size_t kSlotBytes = 2 * kIn * 4 + 2 * 4;
+
 struct Model {
   std::mutex mu;
   bool loaded = false;
@@ -306,6 +314,14 @@ struct Model {
   float *d_buf = nullptr;      // params, norm, x, pred, best
   std::map<std::string, int> cache;
   std::string path;
+  struct Pending {
+    int slot;
+    std::string keys[2]; // pack, unpack
+  };
+  char *h_slots = nullptr;     // pinned: kSlots x kSlotBytes
+  hipEvent_t ev[kSlots] = {};
+  std::deque<Pending> pending; // oldest first
+  std::vector<int> free_slots;
 };
 
 Model &model() {
@@ -397,6 +413,59 @@ static int model_infer(Model &m, const float *x, int n, int *best,
   return TEMPI_SUCCESS;
 }
 
+// Move the picks of finished commit-time inferences into the shape cache,
+// oldest first; `wait` waits for each (else stops at the first still
+// running); `keep` false drops the picks (the weights they came from are
+// being replaced).  The model's lock is held.
+static void harvest(Model &m, bool wait, bool keep = true) {
+  while (!m.pending.empty()) {
+    Model::Pending &p = m.pending.front();
+    const hipError_t e = wait ? hipEventSynchronize(m.ev[p.slot])
+                              : hipEventQuery(m.ev[p.slot]);
+    if (e == hipErrorNotReady) {
+      return;
+    }
+    if (e == hipSuccess && keep) {
+      int picks[2];
+      memcpy(picks, m.h_slots + p.slot * kSlotBytes + 2 * kIn * 4, sizeof picks);
+      for (int d = 0; d < 2; ++d) {
+        if (picks[d] >= 0 && picks[d] < kOut) {
+          m.cache[p.keys[d]] = picks[d];
+        }
+      }
+    } else if (e != hipSuccess) {
+      (void)hipGetLastError();
+    }
+    m.free_slots.push_back(p.slot);
+    m.pending.pop_front();
+  }
+}
+
+// pinned slots and events for commit-time inferences (once per model)
+static int model_slots(Model &m) {
+  if (m.h_slots) {
+    return TEMPI_SUCCESS;
+  }
+  void *h = nullptr;
+  TEMPI_HIP_CHECK(hipHostMalloc(&h, kSlots * kSlotBytes, hipHostMallocDefault));
+  for (int i = 0; i < kSlots; ++i) {
+    if (hipSuccess != hipEventCreateWithFlags(&m.ev[i], hipEventDisableTiming)) {
+      for (int j = 0; j < i; ++j) {
+        (void)hipEventDestroy(m.ev[j]);
+        m.ev[j] = nullptr;
+      }
+      (void)hipHostFree(h);
+      return set_error(TEMPI_ERR_HIP, "event creation failed");
+    }
+  }
+  m.h_slots = static_cast<char *>(h);
+  m.free_slots.clear();
+  for (int i = kSlots - 1; i >= 0; --i) {
+    m.free_slots.push_back(i);
+  }
+  return TEMPI_SUCCESS;
+}
+
 // `name` in the directory the library was loaded from
 static std::string library_file(const char *name) {
   Dl_info info{};
@@ -469,6 +538,9 @@ static int load_model_file(Model &m, const std::string &path) {
   if (!ok) {
     return set_error(TEMPI_ERR_IO, "malformed model file %s", path.c_str());
   }
+  // inferences of the old weights still in flight read d_buf: wait for
+  // them and drop their picks
+  harvest(m, /*wait=*/true, /*keep=*/false);
   if (m.d_buf) {
     (void)hipFree(m.d_buf);
     m.d_buf = nullptr;
@@ -509,6 +581,19 @@ int model_variant(const Geom &g, const char *strided, const char *packed,
   }
   std::lock_guard<std::mutex> lk(m->mu);
   auto it = m->cache.find(key);
+  if (it == m->cache.end() && !m->pending.empty()) {
+    // the commit-time inference of this shape may still be in flight
+    harvest(*m, /*wait=*/false);
+    it = m->cache.find(key);
+    bool in_flight = false;
+    for (const Model::Pending &p : m->pending) {
+      in_flight |= p.keys[0] == key || p.keys[1] == key;
+    }
+    if (it == m->cache.end() && in_flight) {
+      harvest(*m, /*wait=*/true);
+      it = m->cache.find(key);
+    }
+  }
   if (it != m->cache.end()) {
     return it->second;
   }
@@ -551,8 +636,12 @@ int model_ranked(const Geom &g, const char *strided, const char *packed,
   return n;
 }
 
-// commit-time evaluation: the type at incount 1 with 256-B aligned buffers,
-// both directions, in one inference launch; results seed the shape cache
+// Commit-time evaluation: the type at incount 1 with 256-B aligned buffers,
+// both directions, in one inference launch whose picks seed the shape cache.
+// The launch is not waited for: its inputs and picks sit in a pinned slot
+// until the first lookup of the shape (or a later commit) finds its event
+// fired, so MPI_Type_commit does not wait for the GPU -- nor for copies
+// already queued on the library stream.
 void model_precommit(const tempi_strided_block &sb) {
   Model *m = loaded_model();
   if (!m || !have_device() || ensure_runtime()) {
@@ -565,19 +654,58 @@ void model_precommit(const tempi_strided_block &sb) {
   // a 256-aligned stand-in for the buffer origin
   const char *base = reinterpret_cast<const char *>((uintptr_t)1 << 20);
   char *pk = reinterpret_cast<char *>((uintptr_t)1 << 21);
-  float f[2 * kIn];
-  int best[2] = {-1, -1};
-  copy_features(g, base, pk, 0, f);
-  copy_features(g, base, pk, 1, f + kIn);
   std::lock_guard<std::mutex> lk(m->mu);
-  if (model_infer(*m, f, 2, best, nullptr)) {
+  if (model_device_buffers(*m) || model_slots(*m)) {
     return;
   }
-  for (int d = 0; d < 2; ++d) {
-    if (best[d] >= 0 && best[d] < kOut) {
-      m->cache[shape_key_of(g, base, pk, d)] = best[d];
-    }
+  harvest(*m, /*wait=*/false);
+  if (m->free_slots.empty()) {
+    // every slot in flight: wait for the oldest
+    const Model::Pending oldest = m->pending.front();
+    (void)hipEventSynchronize(m->ev[oldest.slot]);
+    harvest(*m, /*wait=*/false);
   }
+  if (m->free_slots.empty()) {
+    return;
+  }
+  const int slot = m->free_slots.back();
+  char *h = m->h_slots + slot * kSlotBytes;
+  void *d = nullptr;
+  if (!device_address(h, &d)) {
+    return;
+  }
+  float *hf = reinterpret_cast<float *>(h);
+  copy_features(g, base, pk, 0, hf);
+  copy_features(g, base, pk, 1, hf + kIn);
+  int *hb = reinterpret_cast<int *>(hf + 2 * kIn);
+  hb[0] = hb[1] = -1;
+  // the kernel reads the features from and writes the picks to the pinned
+  // slot through its device mapping: one launch, no copies
+  const float *df = static_cast<const float *>(d);
+  int *db = reinterpret_cast<int *>(static_cast<char *>(d) + 2 * kIn * 4);
+  hipStream_t s = library_stream();
+  hipLaunchKernelGGL(k_mlp_infer, dim3(1), dim3(kThreads), 0, s, m->d_buf,
+                     m->d_buf + kNParams, df, 2, nullptr, db);
+  if (hipSuccess != hipGetLastError() ||
+      hipSuccess != hipEventRecord(m->ev[slot], s)) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(s); // the slot's buffers may still be in use
+    return;
+  }
+  ++counters().model_inferences;
+  m->free_slots.pop_back();
+  Model::Pending p;
+  p.slot = slot;
+  p.keys[0] = shape_key_of(g, base, pk, 0);
+  p.keys[1] = shape_key_of(g, base, pk, 1);
+  m->pending.push_back(std::move(p));
+}
+
+// at MPI_Finalize: wait for commit-time inferences still in flight
+void model_finalize() {
+  Model &m = model();
+  std::lock_guard<std::mutex> lk(m.mu);
+  harvest(m, /*wait=*/true);
 }
 
 // ---------------------------------------------------------------------------
