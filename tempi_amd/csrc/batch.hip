@@ -441,6 +441,47 @@ int build_batch(tempi_batch *b, const std::vector<Req> &rs) {
   return TEMPI_SUCCESS;
 }
 
+// Tiles of a resolved request (one workgroup pass each).
+uint64_t tiles_of(const Req &q) {
+  const VariantDesc &v = kVariants[q.variant];
+  const int64_t per_unit = (int64_t)v.P * v.U;
+  const int64_t nseg = std::max<int64_t>(1, (q.g.L / v.W + per_unit - 1) / per_unit);
+  const uint64_t units = (uint64_t)(q.g.rows * nseg);
+  const uint64_t per_tile = (uint64_t)(kBlock / v.P) * units_in_flight(v.U);
+  return (units + per_tile - 1) / per_tile;
+}
+
+// The default policy's batch rule.  The learned selector ranks a copy's
+// variants by the time of the copy alone, where a copy under a megabyte is
+// latency-bound and variants that split it into more tiles win.  In a batch
+// with more tiles than the grid runs at once, throughput decides instead:
+// on MI355X the 1 664-request halo batch (all requests under 1 MiB) packed in
+// 129 us with the closed-form rule's variants against 139 us with the
+// selector's, and unpacked in 174 us against 267 us, while the 4 096 random
+// requests (up to 16 MiB) packed in 8.6 ms with the selector's against 9.2 ms.
+// So in a batch that saturates the grid, requests under 1 MiB whose variant
+// was left to the library take the rule; larger requests, and small batches
+// (the headline's three faces: 9.26 us with the selector's picks against
+// 9.91 us with the rule's), keep the selector's.
+void saturated_batch_policy(const tempi_batch_request *reqs,
+                            const std::vector<int64_t> &where,
+                            std::vector<Req> *rs) {
+  uint64_t tiles = 0;
+  for (const Req &q : *rs) {
+    tiles += tiles_of(q);
+  }
+  if (tiles <= (uint64_t)device_cu_count() * 8) {
+    return;
+  }
+  for (size_t k = 0; k < rs->size(); ++k) {
+    Req &q = (*rs)[k];
+    const int asked = reqs[where[k]].variant;
+    if ((asked < 0 || asked >= kNumVariants) && q.g.L * q.g.rows < (1ll << 20)) {
+      q.variant = heuristic_variant(q.g, q.ds, q.dp, q.dir);
+    }
+  }
+}
+
 } // namespace
 
 TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
@@ -455,10 +496,10 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   b->variants.assign((size_t)n, -1);
   std::vector<Req> rs;
   std::vector<int64_t> where;
-  // the per-copy policy at every size: on MI355X the learned selector's
-  // picks for requests under 1 MiB ran batches of grid faces 5-28 % faster
-  // than the closed-form rule's (faces of 128^3 - 256^3 float grids) and the
-  // 64-subdomain halo batch within 1-2 %
+  // the per-copy policy first (in small batches the learned selector's
+  // picks for requests under 1 MiB ran the faces of 128^3 - 256^3 float
+  // grids 5-28 % faster than the closed-form rule's), then the batch rule
+  // for batches that saturate the grid (saturated_batch_policy)
   int err = resolve_requests(
       reqs, n,
       [&](const Req &q) {
@@ -469,6 +510,9 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
       &rs, &where, &b->bytes);
   if (err) {
     return err;
+  }
+  if (method == TEMPI_SELECT_DEFAULT) {
+    saturated_batch_policy(reqs, where, &rs);
   }
   for (size_t k = 0; k < rs.size(); ++k) {
     b->variants[(size_t)where[k]] = rs[k].variant;
