@@ -670,15 +670,20 @@ def bench_halos(ta, m, tc, orc, stream):
         off += exp.size
     del got
     wall_g, _ = timed_loop(ta, lambda: b.graph_launch(stream), reps, stream)
-    # the same requests with the closed-form rule's variants
+    # the same requests with the closed-form rule's variants and with the
+    # selector's per-copy picks
     br = ta.Batch(reqs, method=ta.SELECT_HEURISTIC)
+    bm = ta.Batch(reqs, method=ta.SELECT_MODEL)
     for _ in range(5):
         br.launch(stream)
+        bm.launch(stream)
     _, dev_r = timed_loop(ta, lambda: br.launch(stream), reps, stream)
+    _, dev_m = timed_loop(ta, lambda: bm.launch(stream), reps, stream)
     picks = {}
     for i in range(26):
-        picks[str(i)] = [b.variant(i), br.variant(i)]
+        picks[str(i)] = [b.variant(i), br.variant(i), bm.variant(i)]
     br.destroy()
+    bm.destroy()
 
     def per_call():
         o = 0
@@ -710,7 +715,8 @@ def bench_halos(ta, m, tc, orc, stream):
         "batched_us_per_launch": round(wall_b * 1e6 / reps, 2),
         "batch_kernel_us": round(dev_b * 1e6 / reps, 2),
         "batch_kernel_us_rule_variants": round(dev_r * 1e6 / reps, 2),
-        "variants_default_vs_rule_first_subdomain": picks,
+        "batch_kernel_us_selector_variants": round(dev_m * 1e6 / reps, 2),
+        "variants_default_rule_selector_first_subdomain": picks,
         "batch_kernel_hbm_gbs": round(2 * total / (dev_b / reps) / 1e9, 1),
         "bit_exact": exact,
     }
@@ -797,6 +803,10 @@ def bench_random_requests(ta, tc, orc, stream, nreq=4096, seed=4):
     br.launch(stream)
     _, dev_r = timed_loop(ta, lambda: br.launch(stream), reps, stream)
     br.destroy()
+    bm = ta.Batch(reqs, method=ta.SELECT_MODEL)
+    bm.launch(stream)
+    _, dev_m = timed_loop(ta, lambda: bm.launch(stream), reps, stream)
+    bm.destroy()
     b.launch(stream)
     ta.synchronize(stream)
     got = packed.download()
@@ -824,6 +834,7 @@ def bench_random_requests(ta, tc, orc, stream, nreq=4096, seed=4):
            "batch_kernel_hbm_gbs": round(2 * total / (dev_b / reps) / 1e9, 1),
            "batch_kernel_us": round(dev_b * 1e6 / reps, 2),
            "batch_kernel_us_rule_variants": round(dev_r * 1e6 / reps, 2),
+           "batch_kernel_us_selector_variants": round(dev_m * 1e6 / reps, 2),
            "bit_exact": exact}
     res["speedup"] = round(res["batched_packs_per_s"] / res["per_call_sync_packs_per_s"], 1)
     log(f"random requests: {res}")
