@@ -198,13 +198,6 @@ int pack_or_unpack(const Canon &cn, int count, const char *strided,
 int library_staged(bool pack, const void *typed, int count,
                    MPI_Datatype datatype, void *packed, int packed_size,
                    int *position, MPI_Comm comm) {
-  int bytes = 0;
-  if (MPI_SUCCESS != MPI_Pack_size(count, datatype, comm, &bytes) ||
-      *position < 0 || (int64_t)*position + bytes > (int64_t)packed_size) {
-    // the library's own size check would not see the staged buffers' sizes
-    MPI_Comm_call_errhandler(comm, MPI_ERR_TRUNCATE);
-    return MPI_ERR_TRUNCATE;
-  }
   if (ensure_runtime()) {
     return MPI_ERR_INTERN;
   }
@@ -219,43 +212,57 @@ int library_staged(bool pack, const void *typed, int count,
     }
     typed_at = image.base();
   }
-  const bool packed_dev = device_only(packed);
-  HostStage stage;
-  char *packed_at = static_cast<char *>(packed) + *position;
-  hipStream_t s = library_stream();
-  if (packed_dev) {
+  int err = MPI_SUCCESS;
+  if (!device_only(packed)) {
+    // packed bytes the library can address: it checks sizes and positions
+    err = pack ? real_mpi().Pack(typed_at, count, datatype, packed,
+                                 packed_size, position, comm)
+               : real_mpi().Unpack(packed, packed_size, position, typed_at,
+                                   count, datatype, comm);
+  } else {
+    // packed bytes in device memory cross through a pinned host buffer of
+    // the bytes this call packs (or at most as many as remain to unpack)
+    int bytes = 0;
+    if (MPI_SUCCESS != MPI_Pack_size(count, datatype, comm, &bytes) ||
+        *position < 0 || *position > packed_size ||
+        (pack && (int64_t)*position + bytes > (int64_t)packed_size)) {
+      MPI_Comm_call_errhandler(comm, MPI_ERR_TRUNCATE);
+      return MPI_ERR_TRUNCATE;
+    }
+    if (!pack) {
+      bytes = std::min(bytes, packed_size - *position);
+    }
+    HostStage stage;
     stage.p = pool_acquire(true, (size_t)std::max(bytes, 1));
     if (!stage.p) {
       return MPI_ERR_NO_MEM;
     }
-    if (!pack &&
-        (hipSuccess != hipMemcpyAsync(stage.p, packed_at, (size_t)bytes,
-                                      hipMemcpyDeviceToHost, s) ||
-         hipSuccess != hipStreamSynchronize(s))) {
+    char *dev_at = static_cast<char *>(packed) + *position;
+    hipStream_t s = library_stream();
+    if (!pack && (hipSuccess != hipMemcpyAsync(stage.p, dev_at, (size_t)bytes,
+                                               hipMemcpyDeviceToHost, s) ||
+                  hipSuccess != hipStreamSynchronize(s))) {
       return MPI_ERR_INTERN;
     }
-    packed_at = static_cast<char *>(stage.p);
+    int pos = 0;
+    err = pack ? real_mpi().Pack(typed_at, count, datatype, stage.p, bytes,
+                                 &pos, comm)
+               : real_mpi().Unpack(stage.p, bytes, &pos, typed_at, count,
+                                   datatype, comm);
+    if (err != MPI_SUCCESS) {
+      return err;
+    }
+    if (pack && (hipSuccess != hipMemcpyAsync(dev_at, stage.p, (size_t)pos,
+                                              hipMemcpyHostToDevice, s) ||
+                 hipSuccess != hipStreamSynchronize(s))) {
+      return MPI_ERR_INTERN;
+    }
+    *position += pos;
   }
-  int pos = 0;
-  int err = pack ? real_mpi().Pack(typed_at, count, datatype, packed_at, bytes,
-                                   &pos, comm)
-                 : real_mpi().Unpack(packed_at, bytes, &pos, typed_at, count,
-                                     datatype, comm);
-  if (err != MPI_SUCCESS) {
-    return err;
-  }
-  if (pack && packed_dev &&
-      (hipSuccess != hipMemcpyAsync(static_cast<char *>(packed) + *position,
-                                    stage.p, (size_t)pos,
-                                    hipMemcpyHostToDevice, s) ||
-       hipSuccess != hipStreamSynchronize(s))) {
+  if (err == MPI_SUCCESS && !pack && image.host && image.put_back()) {
     return MPI_ERR_INTERN;
   }
-  if (!pack && image.host && image.put_back()) {
-    return MPI_ERR_INTERN;
-  }
-  *position += pos;
-  return MPI_SUCCESS;
+  return err;
 }
 
 // would the MPI library be handed device memory it cannot address?

@@ -447,7 +447,10 @@ static int model_slots(Model &m) {
     return TEMPI_SUCCESS;
   }
   void *h = nullptr;
-  TEMPI_HIP_CHECK(hipHostMalloc(&h, kSlots * kSlotBytes, hipHostMallocDefault));
+  // mapped like the ONE_SHOT staging buffers: the kernel reads and writes
+  // the slots through their device address
+  TEMPI_HIP_CHECK(hipHostMalloc(&h, kSlots * kSlotBytes,
+                                hipHostMallocMapped | hipHostMallocPortable));
   for (int i = 0; i < kSlots; ++i) {
     if (hipSuccess != hipEventCreateWithFlags(&m.ev[i], hipEventDisableTiming)) {
       for (int j = 0; j < i; ++j) {
