@@ -338,19 +338,25 @@ bool type_lookup(MPI_Datatype t, Canon *out);
 int type_span(MPI_Datatype dt, int count, int64_t *lo, int64_t *hi);
 // A pinned host image of the type-map span of `count` objects of `dt` in
 // device memory, for handing typed device data to an MPI library that
-// cannot read device memory (interpose.cpp).  take() copies the span in
-// when asked; put_back() copies the whole span back (so the bytes between
-// blocks keep the values they had when it was taken).
+// cannot read device memory (interpose.cpp).  take() copies the span in;
+// for an image the library will write (a receive, an unpack) it also keeps
+// a snapshot, and put_back() writes to the device only the bytes that
+// differ from it -- the bytes the library changed.  Bytes between the type
+// map's blocks are never written back, so device writes to them made while
+// the image was out (a kernel updating a subdomain's interior while its
+// ghost cells are received) are kept.
 struct SpanImage {
-  char *host = nullptr; // pinned host memory, `len` bytes
-  char *dev = nullptr;  // first byte of the span on the device
-  int64_t lo = 0;       // span start from the buffer origin
+  char *host = nullptr;   // pinned host memory, `len` bytes
+  char *before = nullptr; // the span as taken (images the library writes)
+  char *dev = nullptr;    // first byte of the span on the device
+  int64_t lo = 0;         // span start from the buffer origin
   int64_t len = 0;
   SpanImage() = default;
   SpanImage(const SpanImage &) = delete;
   SpanImage &operator=(const SpanImage &) = delete;
   ~SpanImage();
-  int take(const void *dev_buf, MPI_Datatype dt, int count, bool copy_in);
+  // `written`: the library writes into the image (put_back follows)
+  int take(const void *dev_buf, MPI_Datatype dt, int count, bool written);
   int put_back();
   // the buffer origin as the MPI library sees it
   void *base() const { return host - lo; }

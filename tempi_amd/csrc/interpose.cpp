@@ -75,10 +75,52 @@ int type_span(MPI_Datatype dt, int count, int64_t *lo, int64_t *hi) {
   return TEMPI_SUCCESS;
 }
 
-SpanImage::~SpanImage() { pool_release(true, host); }
+namespace {
+
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+// Writes to dst[0, n) the bytes of `after` that differ from `before` (both
+// 16-B aligned, read through the device mapping of pinned host memory):
+// 16 bytes compared per thread and step, byte stores where they differ.
+__global__ void __launch_bounds__(256)
+    k_write_changed(char *dst, const char *__restrict__ after,
+                    const char *__restrict__ before, int64_t n) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x * 16;
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+       i < n; i += step) {
+    if (n - i >= 16) {
+      const u32x4_t a = *reinterpret_cast<const u32x4_t *>(after + i);
+      const u32x4_t b = *reinterpret_cast<const u32x4_t *>(before + i);
+      if (a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w) {
+        continue;
+      }
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const uint32_t wa = a[k >> 2] >> (8 * (k & 3));
+        const uint32_t wb = b[k >> 2] >> (8 * (k & 3));
+        if ((uint8_t)wa != (uint8_t)wb) {
+          dst[i + k] = (char)wa;
+        }
+      }
+    } else {
+      for (int64_t k = i; k < n; ++k) {
+        if (after[k] != before[k]) {
+          dst[k] = after[k];
+        }
+      }
+    }
+  }
+}
+
+} // namespace
+
+SpanImage::~SpanImage() {
+  pool_release(true, host);
+  pool_release(true, before);
+}
 
 int SpanImage::take(const void *dev_buf, MPI_Datatype dt, int count,
-                    bool copy_in) {
+                    bool written) {
   int64_t hi = 0;
   int err = type_span(dt, count, &lo, &hi);
   if (err) {
@@ -89,26 +131,44 @@ int SpanImage::take(const void *dev_buf, MPI_Datatype dt, int count,
   if ((err = ensure_runtime())) {
     return err;
   }
-  host = static_cast<char *>(pool_acquire(true, (size_t)std::max<int64_t>(len, 1)));
-  if (!host) {
+  const size_t bytes = (size_t)std::max<int64_t>(len, 1);
+  host = static_cast<char *>(pool_acquire(true, bytes));
+  if (written && host) {
+    before = static_cast<char *>(pool_acquire(true, bytes));
+  }
+  if (!host || (written && !before)) {
     return set_error(TEMPI_ERR_NOMEM, "host image of %lld B", (long long)len);
   }
-  if (copy_in && len > 0) {
+  if (len > 0) {
     hipStream_t s = library_stream();
     TEMPI_HIP_CHECK(hipMemcpyAsync(host, dev, (size_t)len,
                                    hipMemcpyDeviceToHost, s));
+    if (written) {
+      TEMPI_HIP_CHECK(hipMemcpyAsync(before, dev, (size_t)len,
+                                     hipMemcpyDeviceToHost, s));
+    }
     TEMPI_HIP_CHECK(hipStreamSynchronize(s));
   }
   return TEMPI_SUCCESS;
 }
 
 int SpanImage::put_back() {
-  if (len > 0) {
-    hipStream_t s = library_stream();
-    TEMPI_HIP_CHECK(hipMemcpyAsync(dev, host, (size_t)len,
-                                   hipMemcpyHostToDevice, s));
-    TEMPI_HIP_CHECK(hipStreamSynchronize(s));
+  if (len <= 0) {
+    return TEMPI_SUCCESS;
   }
+  void *a = nullptr, *b = nullptr;
+  if (!before || !device_address(host, &a) || !device_address(before, &b)) {
+    return set_error(TEMPI_ERR_ARG, "host image was not taken for writing");
+  }
+  hipStream_t s = library_stream();
+  const int64_t threads = (len + 15) / 16;
+  const int64_t blocks = std::min<int64_t>((threads + 255) / 256,
+                                           (int64_t)device_cu_count() * 8);
+  hipLaunchKernelGGL(k_write_changed, dim3((unsigned)blocks), dim3(256), 0, s,
+                     dev, static_cast<const char *>(a),
+                     static_cast<const char *>(b), len);
+  TEMPI_HIP_CHECK(hipGetLastError());
+  TEMPI_HIP_CHECK(hipStreamSynchronize(s));
   return TEMPI_SUCCESS;
 }
 
@@ -191,8 +251,9 @@ int pack_or_unpack(const Canon &cn, int count, const char *strided,
 // are host data on one side and device memory on the other, or packing is
 // switched off.  A device-resident typed buffer is replaced by a pinned
 // host image of its type map's span ([true lb, true ub) over the objects);
-// after an unpack the whole image is copied back, so the bytes between the
-// type map's blocks keep the values they had.  Device-resident packed
+// after an unpack only the bytes the library changed in the image are
+// written back, so the bytes between the type map's blocks are never
+// written.  Device-resident packed
 // bytes cross through a pinned host buffer.  Without this the library would
 // be handed a device pointer.  MPI error code.
 int library_staged(bool pack, const void *typed, int count,
@@ -205,7 +266,7 @@ int library_staged(bool pack, const void *typed, int count,
   SpanImage image;
   void *typed_at = const_cast<void *>(typed);
   if (device_only(typed)) {
-    if (image.take(typed, datatype, count, /*copy_in=*/true)) {
+    if (image.take(typed, datatype, count, /*written=*/!pack)) {
       TEMPI_LOG(LOG_ERROR, "staging for the MPI library: %s",
                 tempi_last_error());
       return MPI_ERR_NO_MEM;

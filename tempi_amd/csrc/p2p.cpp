@@ -270,8 +270,9 @@ public:
 // Typed device data whose type has no strided form, handed to an MPI
 // library that cannot read device memory: the library sends from (or
 // receives into) a pinned host image of the type map's span, taken at the
-// start; a receive copies the image back when it completes, so the bytes
-// between the type map's blocks keep their values.
+// start; when a receive completes, only the bytes the library changed in
+// the image are written to the device, so writes to the bytes between the
+// type map's blocks made while the receive was pending are kept.
 class SpanOp : public AsyncOp {
   bool send_;
   bool done_ = false;
@@ -298,7 +299,7 @@ public:
   }
   int start(const void *buf, int count, MPI_Datatype dt, int peer, int tag,
             MPI_Comm comm) {
-    if (img_.take(buf, dt, count, /*copy_in=*/true)) {
+    if (img_.take(buf, dt, count, /*written=*/!send_)) {
       TEMPI_LOG(LOG_ERROR, "staging device data: %s", tempi_last_error());
       return MPI_ERR_NO_MEM;
     }
@@ -786,7 +787,7 @@ TEMPI_EXPORT int MPI_Send(const void *buf, int count, MPI_Datatype datatype,
     }
     if (library_cannot_read(buf, count, dest)) {
       SpanImage img;
-      if (img.take(buf, datatype, count, /*copy_in=*/true)) {
+      if (img.take(buf, datatype, count, /*written=*/false)) {
         TEMPI_LOG(LOG_ERROR, "MPI_Send: %s", tempi_last_error());
         return MPI_ERR_NO_MEM;
       }
@@ -841,7 +842,7 @@ TEMPI_EXPORT int MPI_Recv(void *buf, int count, MPI_Datatype datatype,
     }
     if (library_cannot_read(buf, count, source)) {
       SpanImage img;
-      if (img.take(buf, datatype, count, /*copy_in=*/true)) {
+      if (img.take(buf, datatype, count, /*written=*/true)) {
         TEMPI_LOG(LOG_ERROR, "MPI_Recv: %s", tempi_last_error());
         return MPI_ERR_NO_MEM;
       }

@@ -550,6 +550,48 @@ def test_device_data_without_strided_form_sent_through_host_images(mpi):
     mpi.free(h)
 
 
+def test_pending_receive_keeps_device_writes_between_the_blocks(mpi):
+    """A receive of a two-block struct into device memory (host image, the
+    library cannot read device memory) is posted; while it is pending every
+    byte between the type map's blocks is rewritten on the device (a kernel
+    updating the rest of the buffer, as in a halo exchange overlapped with
+    interior work).  Completion writes the received type map and nothing
+    else: the rewritten bytes keep their new values."""
+    h = mpi.type_from_spec(STRUCT2)
+    lb, ext = mpi.extent(h)
+    count = 4
+    n = ext * count
+    size = mpi.pack_size(count, h)
+    # the type map's bytes: MPICH unpacks all-ones packed bytes into zeros
+    ones = np.full(size, 0xFF, np.uint8)
+    tm = np.zeros(n, np.uint8)
+    assert mpi.unpack(ones.ctypes.data, size, tm.ctypes.data, count, h, library=True)[0] == 0
+    in_map = tm != 0
+    assert in_map.sum() == size and (~in_map).sum() > 0
+    host = orc.splitmix_bytes(n, 47)
+    packed = np.zeros(size, np.uint8)
+    assert mpi.pack(host.ctypes.data, count, h, packed.ctypes.data, size, library=True)[0] == 0
+    received = np.zeros(n, np.uint8)
+    assert mpi.unpack(packed.ctypes.data, size, received.ctypes.data, count, h,
+                      library=True)[0] == 0
+    expected = np.where(in_map, received, 0xCD).astype(np.uint8)
+    src = ta.DeviceBuffer(n)
+    src.upload(host)
+    dst = ta.DeviceBuffer(n)
+    dst.fill(0xAB)
+    err, rr = mpi.irecv(dst.addr(), count, h, 0, 35)
+    assert err == 0
+    dst.upload(np.where(in_map, 0xAB, 0xCD).astype(np.uint8))
+    err, sr = mpi.isend(src.addr(), count, h, 0, 35)
+    assert err == 0
+    err, sts, _ = mpi.waitall([rr, sr], [h, h])
+    assert err == 0 and sts[0]["count"] == count
+    got = dst.download()
+    assert (got[~in_map] == 0xCD).all()
+    assert (got == expected).all()
+    mpi.free(h)
+
+
 def test_contiguous_device_data_is_staged_without_gpu_aware_library(mpi):
     """MPI_FLOAT device buffers (contiguous) are staged through the host
     instead of reaching a library that cannot read device memory."""
