@@ -152,6 +152,15 @@ def test_supported_extras(mpi):
     check_type(mpi, ("hvector", 4, 2, -24, ("contiguous", 3, tc.BYTE)))
 
 
+def test_negative_strides(mpi):
+    """Vectors and hvectors with negative strides lower to strided blocks
+    with negative strides (no reordering: packed order is type-map order)
+    that gather MPICH's bytes at incounts 1-3."""
+    for spec, form in tc.NEGATIVE_STRIDES:
+        sb = check_type(mpi, spec)
+        assert sb.as_tuple() == form, spec
+
+
 def test_equivalent_constructions_share_one_canonical_form(mpi):
     """The reference's type_equivalence test (test/type_equivalence.cpp:51-61)
     only compares sizes; here every construction of the same 3D layout must

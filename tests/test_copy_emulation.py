@@ -15,6 +15,7 @@ import random
 import pytest
 
 import tempi_amd as ta
+import typecatalog as tc
 
 SB = ta.StridedBlock.make
 BASE_S = 1 << 40  # addresses only: nothing is dereferenced
@@ -109,6 +110,18 @@ def test_long_rows_and_negative_strides():
             sb = SB(3 * 6000, 6000, [L, 3], [1, -6000])
             for direction in (ta.DIR_PACK, ta.DIR_UNPACK):
                 check_copy(sb, 2, s_off, rng.randrange(16), direction, variant)
+
+
+def test_negative_stride_catalog():
+    """The strided blocks the negative-stride vectors and hvectors lower to
+    (blocks below the origin, reverse address order): every variant, both
+    directions, incount 1 and 3, misaligned origins."""
+    for spec, (start, extent, counts, strides) in tc.NEGATIVE_STRIDES:
+        sb = SB(start, extent, list(counts), list(strides))
+        for variant in range(ta.num_variants()):
+            for count in (1, 3):
+                for direction in (ta.DIR_PACK, ta.DIR_UNPACK):
+                    check_copy(sb, count, 5, 3, direction, variant)
 
 
 def test_units_past_two_to_the_31_are_split():

@@ -577,3 +577,46 @@ def test_predefined_types_on_device_take_the_kernel_path(mpi):
     after = ta.counters()
     assert after["pack_kernel_launches"] == before["pack_kernel_launches"] + 1
     assert after["library_pack_calls"] == before["library_pack_calls"]
+
+
+@pytest.mark.parametrize("k", range(len(tc.NEGATIVE_STRIDES)))
+def test_negative_strides_on_device(mpi, k):
+    """Vectors and hvectors with negative strides (blocks below the buffer
+    origin, in reverse address order) pack and unpack on device buffers
+    through the kernels: packed bytes equal MPICH's on a host copy, unpack
+    equals MPICH's unpack into 0xAB (bytes between blocks keep 0xAB)."""
+    spec, _ = tc.NEGATIVE_STRIDES[k]
+    h = mpi.type_from_spec(spec)
+    assert mpi.strided_block(h) is not None
+    o = orc.OracleType(spec)
+    for incount in (1, 3):
+        lo, hi = o.span(incount)
+        origin = -min(lo, 0) + 16
+        length = origin + hi + 16
+        host = orc.splitmix_bytes(length, 53 + k)
+        size = mpi.pack_size(incount, h)
+        expected = np.zeros(size, np.uint8)
+        err, pos = mpi.pack(host.ctypes.data + origin, incount, h, expected.ctypes.data, size,
+                            library=True)
+        assert err == 0 and pos == size
+        src = ta.DeviceBuffer(length)
+        src.upload(host)
+        packed = ta.DeviceBuffer(size)
+        before = ta.counters()
+        err, pos = mpi.pack(src.addr(origin), incount, h, packed.addr(), size)
+        assert err == 0 and pos == size
+        assert (packed.download() == expected).all(), (spec, incount)
+        ref = np.full(length, 0xAB, np.uint8)
+        err, _ = mpi.unpack(expected.ctypes.data, size, ref.ctypes.data + origin, incount, h,
+                            library=True)
+        assert err == 0
+        dst = ta.DeviceBuffer(length)
+        dst.fill(0xAB)
+        err, pos = mpi.unpack(packed.addr(), size, dst.addr(origin), incount, h)
+        assert err == 0 and pos == size
+        assert (dst.download() == ref).all(), (spec, incount)
+        after = ta.counters()
+        assert after["pack_kernel_launches"] == before["pack_kernel_launches"] + 1
+        assert after["unpack_kernel_launches"] == before["unpack_kernel_launches"] + 1
+        assert after["library_pack_calls"] == before["library_pack_calls"]
+    mpi.free(h)

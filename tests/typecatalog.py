@@ -151,6 +151,20 @@ def grid_face_subarray(n, ghost, face, elem=FLOAT):
 # seeded random nested types
 # ---------------------------------------------------------------------------
 
+# Vectors and hvectors with negative strides (blocks in reverse address
+# order; MPI allows them): the type map starts at the origin and reaches
+# below it.  With the strided block each lowers to, as (start, extent,
+# counts, strides).
+NEGATIVE_STRIDES = [
+    (("vector", 5, 2, -3, FLOAT), (0, 56, (8, 5), (1, -12))),
+    (("hvector", 4, 3, -40, ("vector", 2, 1, 3, INT)), (0, 168, (4, 2, 3, 4), (1, 12, 16, -40))),
+    (("vector", 3, 1, -2, ("hvector", 4, 2, -12, BYTE)), (0, 190, (2, 4, 3), (1, -12, -76))),
+    (("hvector", 6, 1, -1000, ("contiguous", 100, BYTE)), (0, 5100, (100, 6), (1, -1000))),
+    (("vector", 64, 3, -5, DOUBLE), (0, 2544, (24, 64), (1, -40))),
+    (("hvector", 3, 2, -7, ("subarray", [4, 6], [2, 3], [1, 2], "C", FLOAT)),
+     (32, 206, (12, 2, 2, 3), (1, 24, 96, -7))),
+]
+
 _NAMED = [BYTE, BYTE, BYTE, FLOAT, DOUBLE, INT, ("named", "MPI_SHORT")]
 
 
