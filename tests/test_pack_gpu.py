@@ -620,3 +620,38 @@ def test_negative_strides_on_device(mpi, k):
         assert after["unpack_kernel_launches"] == before["unpack_kernel_launches"] + 1
         assert after["library_pack_calls"] == before["library_pack_calls"]
     mpi.free(h)
+
+
+def test_empty_packs_leave_everything_untouched(mpi):
+    """Nothing to move -- incount 0, a zero-count vector, the C ABI at
+    incount 0 -- succeeds on device buffers without a copy: the position
+    stays where it was and no byte changes, as MPICH does on host buffers."""
+    h = mpi.type_from_spec(("vector", 4, 2, 5, tc.FLOAT))
+    empty = mpi.type_from_spec(("vector", 0, 1, 1, tc.FLOAT))
+    assert mpi.size(empty) == 0
+    src = ta.DeviceBuffer(256)
+    src.fill(0x11)
+    packed = ta.DeviceBuffer(64)
+    packed.fill(0x5A)
+    host_src = np.full(256, 0x11, np.uint8)
+    host_packed = np.full(64, 0x5A, np.uint8)
+    before = ta.counters()
+    for t, count in ((h, 0), (empty, 3)):
+        err, pos = mpi.pack(src.addr(), count, t, packed.addr(), 64, position=5)
+        assert err == 0 and pos == 5, (count, err, pos)
+        err, lpos = mpi.pack(host_src.ctypes.data, count, t, host_packed.ctypes.data, 64,
+                             position=5, library=True)
+        assert err == 0 and lpos == 5
+        err, pos = mpi.unpack(packed.addr(), 64, src.addr(), count, t, position=7)
+        assert err == 0 and pos == 7
+    sb = mpi.strided_block(h)[0]
+    assert ta.pack_sb(sb, src.addr(), 0, packed.addr(), 64, position=9) == 9
+    assert ta.unpack_sb(sb, packed.addr(), 64, src.addr(), 0, position=11) == 11
+    ta.synchronize()
+    after = ta.counters()
+    assert after["pack_kernel_launches"] == before["pack_kernel_launches"]
+    assert after["unpack_kernel_launches"] == before["unpack_kernel_launches"]
+    assert (packed.download() == 0x5A).all() and (src.download() == 0x11).all()
+    assert (host_packed == 0x5A).all()
+    mpi.free(h)
+    mpi.free(empty)
