@@ -92,7 +92,10 @@ int flush_caches_on(int64_t bytes, hipStream_t s) {
       f.bytes = 0;
     }
     TEMPI_HIP_CHECK(hipMalloc(&f.buf, (size_t)bytes + blocks * 4));
-    TEMPI_HIP_CHECK(hipMemset(f.buf, 1, (size_t)bytes));
+    // filled on the sweep's own stream: a fill on the null stream is not
+    // ordered with this (non-blocking) stream and ran on under the first
+    // sweep and the copy timed after it (that copy measured 40-70 % slow)
+    TEMPI_HIP_CHECK(hipMemsetAsync(f.buf, 1, (size_t)bytes, s));
     f.bytes = bytes;
   }
   hipLaunchKernelGGL(k_touch, dim3(blocks), dim3(256), 0, s,
