@@ -82,6 +82,10 @@ def main():
     ap.add_argument("--profile-sweep", action="store_true",
                     help="the sweep's copies of >= 64-B blocks and >= 16 MiB alone, each pack "
                          "and unpack launched 3 times after a cache sweep (for rocprofv3)")
+    ap.add_argument("--profile-random", action="store_true",
+                    help="only the 4096 random requests as one batch, three launches back "
+                         "to back and three after a cache sweep, then the 256 MiB "
+                         "contiguous calibration copy (for rocprofv3)")
     ap.add_argument("--profile-halo", action="store_true",
                     help="only the halo batch (1664 requests), warm and cold launches, then "
                          "the 256 MiB contiguous calibration copy (for rocprofv3)")
@@ -110,6 +114,9 @@ def main():
         return
     if args.profile_halo:
         profile_halo(ta, m, tc, stream)
+        return
+    if args.profile_random:
+        profile_random(ta, tc, stream)
         return
     if args.profile_sweep:
         profile_sweep(ta, m, tc, stream)
@@ -642,6 +649,54 @@ def profile_halo(ta, m, tc, stream, warm=5, cold=3):
                stream)
     ta.synchronize(stream)
     for x in keep + [a, c]:
+        x.free()
+
+
+def profile_random(ta, tc, stream, warm=3, cold=3):
+    """The 4096 random requests of bench_random_requests as one batch (for
+    rocprofv3): `warm` launches back to back, then `cold` launches each after
+    a read-only 1 GiB sweep, then one 256 MiB contiguous copy (variant 1, the
+    FETCH_SIZE / WRITE_SIZE calibration).  No request reaches the size at
+    which the library times candidates, so the batch's creation launches no
+    copy kernel and the k_batch dispatches of a trace are these launches, in
+    order; one JSON line each, with the batch's algorithmic bytes and the
+    128-B lines its requests touch (summed over requests: they all read one
+    source buffer, so lines shared between requests count once per
+    request)."""
+    geo, span_max = tc.random_requests(4, 4096)
+    sbs = [ta.StridedBlock.make(*g) for g in geo]
+    total = sum(sb.size() for sb in sbs)
+    src = ta.DeviceBuffer(span_max)
+    src.fill(7)
+    packed = ta.DeviceBuffer(total)
+    reqs, off, lines = [], 0, 0
+    for sb in sbs:
+        reqs.append((sb, src.addr(), packed.addr(off), 1, ta.DIR_PACK))
+        ls, lp = line_bound(sb, src.addr() % 128)
+        lines += ls + lp
+        off += sb.size()
+    b = ta.Batch(reqs)
+    ev = [ta.Event(), ta.Event()]
+    for i in range(warm + cold):
+        kind = "warm" if i < warm else "cold"
+        if kind == "cold":
+            ta.flush_caches()
+        ta.synchronize(stream)
+        ev[0].record(stream)
+        b.launch(stream)
+        ev[1].record(stream)
+        ta.synchronize(stream)
+        print(json.dumps({"launch": kind, "us": round(ev[0].elapsed_ms(ev[1]) * 1e3, 2),
+                          "requests": len(reqs), "packed_bytes": total,
+                          "lines_touched": lines}), flush=True)
+    b.destroy()
+    cal = 256 << 20
+    a, c = ta.DeviceBuffer(cal), ta.DeviceBuffer(cal)
+    a.fill(1)
+    ta.pack_sb(ta.StridedBlock.make(0, cal, [cal], [1]), a.addr(), 1, c.addr(), cal, 0, 1,
+               stream)
+    ta.synchronize(stream)
+    for x in (src, packed, a, c):
         x.free()
 
 
