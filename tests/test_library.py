@@ -204,3 +204,9 @@ def test_ld_preload_in_front_of_mpi(tmp_path):
     r = subprocess.run([str(exe)], env=env, capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     assert r.stdout.split() == ["0", "1", "4", "5", "8", "9", "6"]
+    # at debug level MPI_Finalize reports every counter (the reference's
+    # counter dump, src/internal/counters.cpp:30-120): here one commit with
+    # a strided form and one host pack forwarded to the MPI library
+    assert "rank 0 counters::commits_canonical: 1" in r.stderr, r.stderr
+    assert "rank 0 counters::library_pack_calls: 1" in r.stderr, r.stderr
+    assert r.stderr.count("rank 0 counters::") == 18
