@@ -137,7 +137,7 @@ def build_test_app(verbose: bool = False) -> str:
         # libmpi by path: a -L of the MPI directory would also resolve
         # libstdc++ from there (conda ships an older one)
         cmd = [hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", src,
-               f"-I{os.path.join(mpi, 'include')}",
+               f"-I{os.path.join(mpi, 'include')}", f"-I{os.path.join(ROOT, 'include')}",
                f"-Wl,{os.path.join(mpi, 'lib', 'libmpi.so')}",
                f"-Wl,-rpath,{':'.join(rpath)}", "-Wl,--enable-new-dtags", "-o", out]
         res = subprocess.run(cmd, capture_output=True, text=True)

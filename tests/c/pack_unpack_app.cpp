@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <mpi.h>
 
+#include "tempi_amd.h" // the counters' layout only: nothing is linked from it
+
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -124,12 +126,13 @@ int main(int argc, char **argv) {
   }
   test_type("vector(1024,256,512)", vec(1024, 256, 512), 1);
   // the preloaded library's counters, if it is there: kernels launched
-  typedef int (*counters_fn)(int64_t *);
+  typedef int (*counters_fn)(tempi_counters *);
   if (counters_fn f = (counters_fn)dlsym(RTLD_DEFAULT, "tempi_counters_get")) {
-    int64_t c[16] = {0};
-    f(c);
+    tempi_counters c{};
+    f(&c);
     printf("pack kernels %lld unpack kernels %lld library packs %lld\n",
-           (long long)c[0], (long long)c[1], (long long)c[4]);
+           (long long)c.pack_kernel_launches, (long long)c.unpack_kernel_launches,
+           (long long)c.library_pack_calls);
   }
   MPI_Finalize();
   printf("%d failures\n", failures);

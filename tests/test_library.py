@@ -210,3 +210,38 @@ def test_ld_preload_in_front_of_mpi(tmp_path):
     assert "rank 0 counters::commits_canonical: 1" in r.stderr, r.stderr
     assert "rank 0 counters::library_pack_calls: 1" in r.stderr, r.stderr
     assert r.stderr.count("rank 0 counters::") == 18
+
+
+STRUCTS = {  # C ABI struct -> (ctypes mirror, its last field)
+    "tempi_strided_block": ("StridedBlock", "strides"),
+    "tempi_variant_info": ("VariantInfo", None),
+    "tempi_copy_emulation": ("CopyEmulation", "reserved"),
+    "tempi_batch_request": ("BatchRequest", "variant"),
+    "tempi_counters": ("Counters", "host_staged_calls"),
+}
+
+
+def test_ctypes_mirrors_match_the_header(tmp_path):
+    """Every struct of include/tempi_amd.h has the size (and last-field
+    offset) its Python ctypes mirror assumes -- a caller's buffer shorter
+    than the struct the library fills is written past its end."""
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "tempi_amd.h"',
+             "int main(void) {"]
+    for c, (_, last) in STRUCTS.items():
+        off = f"offsetof({c}, {last})" if last else "0"
+        lines.append(f'  printf("{c} %zu %zu\\n", sizeof({c}), (size_t){off});')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "sizes"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run(["gcc", str(src), "-o", str(exe), f"-I{root}/include",
+                    "-I/opt/conda/include"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    for line in out.splitlines():
+        c, size, off = line.split()
+        py, last = STRUCTS[c]
+        cls = getattr(ta, py)
+        assert ctypes.sizeof(cls) == int(size), (c, ctypes.sizeof(cls), size)
+        if last:
+            assert getattr(cls, last).offset == int(off), (c, last)
