@@ -314,12 +314,20 @@ __global__ void __launch_bounds__(kThreads)
 // host side
 // ---------------------------------------------------------------------------
 
-// Commit-time inferences run without waiting (model_precommit): each takes
-// a slot of pinned host memory that the kernel reads and writes through its
-// device mapping -- the type's two feature vectors and two picks -- until
-// its event has fired and its picks are in the shape cache.
-constexpr int kSlots = 16;
-constexpr size_t kSlotBytes = 2 * kIn * 4 + 2 * 4;
+// Commit-time inferences run without waiting (model_precommit).  A batch of
+// up to kSlotTypes committed types takes a slot of pinned host memory that
+// the kernel reads and writes through its device mapping -- two feature
+// vectors and two picks per type -- until its event has fired and its picks
+// are in the shape cache.  A commit that finds an inference of ours still
+// running adds its type to the open batch instead of launching one of its
+// own; the batch launches when the GPU is free of ours again (at the next
+// commit), when it is full, or when a lookup needs one of its shapes.
+constexpr int kSlots = 8;
+constexpr int kSlotTypes = 16;
+constexpr int kSlotVecs = 2 * kSlotTypes; // one inference chunk (kB)
+static_assert(kSlotVecs <= kB, "a batch is one inference chunk");
+constexpr size_t kSlotPicks = (size_t)kSlotVecs * kIn * 4; // picks' offset
+constexpr size_t kSlotBytes = kSlotPicks + (size_t)kSlotVecs * 4;
 
 struct Model {
   std::mutex mu;
@@ -332,7 +340,9 @@ struct Model {
   std::string path;
   struct Pending {
     int slot;
-    std::string keys[2]; // pack, unpack
+    int n = 0;                     // types in the batch
+    bool launched = false;         // else the open batch (always the newest)
+    std::vector<std::string> keys; // pack, unpack of each type
   };
   char *h_slots = nullptr;     // pinned: kSlots x kSlotBytes
   hipEvent_t ev[kSlots] = {};
@@ -429,24 +439,71 @@ static int model_infer(Model &m, const float *x, int n, int *best,
   return TEMPI_SUCCESS;
 }
 
+// Launch the open batch (the newest pending one, if not launched yet): one
+// inference over its types' feature vectors, read from and answered into
+// its pinned slot through the slot's device mapping; its event fires when
+// the picks are there.  A batch that cannot be launched is dropped (its
+// shapes are evaluated again when first looked up).  The model's lock is
+// held.
+static void launch_open(Model &m) {
+  if (m.pending.empty() || m.pending.back().launched) {
+    return;
+  }
+  Model::Pending &b = m.pending.back();
+  char *h = m.h_slots + b.slot * kSlotBytes;
+  void *d = nullptr;
+  hipStream_t s = library_stream();
+  bool ok = device_address(h, &d) != 0;
+  if (ok) {
+    const float *df = static_cast<const float *>(d);
+    int *db = reinterpret_cast<int *>(static_cast<char *>(d) + kSlotPicks);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(1), dim3(kThreads), 0, s, m.d_buf,
+                       m.d_buf + kNParams, df, 2 * b.n, nullptr, db);
+    ok = hipSuccess == hipGetLastError() &&
+         hipSuccess == hipEventRecord(m.ev[b.slot], s);
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(s); // the slot's buffers may still be in use
+    m.free_slots.push_back(b.slot);
+    m.pending.pop_back();
+    return;
+  }
+  b.launched = true;
+  ++counters().model_inferences;
+}
+
 // Move the picks of finished commit-time inferences into the shape cache,
-// oldest first; `wait` waits for each (else stops at the first still
-// running); `keep` false drops the picks (the weights they came from are
-// being replaced).  The model's lock is held.
+// oldest first; `wait` waits for each, launching the open batch first (else
+// stops at the first still running, or at the open batch); `keep` false
+// drops the picks, and open batches unlaunched (the weights they came from
+// are being replaced).  The model's lock is held.
 static void harvest(Model &m, bool wait, bool keep = true) {
   while (!m.pending.empty()) {
     Model::Pending &p = m.pending.front();
+    if (!p.launched) {
+      if (!keep) {
+        m.free_slots.push_back(p.slot);
+        m.pending.pop_front();
+        continue;
+      }
+      if (!wait) {
+        return;
+      }
+      launch_open(m);
+      continue; // launched, or dropped
+    }
     const hipError_t e = wait ? hipEventSynchronize(m.ev[p.slot])
                               : hipEventQuery(m.ev[p.slot]);
     if (e == hipErrorNotReady) {
       return;
     }
     if (e == hipSuccess && keep) {
-      int picks[2];
-      memcpy(picks, m.h_slots + p.slot * kSlotBytes + 2 * kIn * 4, sizeof picks);
-      for (int d = 0; d < 2; ++d) {
-        if (picks[d] >= 0 && picks[d] < kOut) {
-          m.cache[p.keys[d]] = picks[d];
+      const int *picks =
+          reinterpret_cast<const int *>(m.h_slots + p.slot * kSlotBytes + kSlotPicks);
+      for (int v = 0; v < 2 * p.n; ++v) {
+        if (picks[v] >= 0 && picks[v] < kOut) {
+          m.cache[p.keys[(size_t)v]] = picks[v];
         }
       }
     } else if (e != hipSuccess) {
@@ -601,14 +658,15 @@ int model_variant(const Geom &g, const char *strided, const char *packed,
   std::lock_guard<std::mutex> lk(m->mu);
   auto it = m->cache.find(key);
   if (it == m->cache.end() && !m->pending.empty()) {
-    // the commit-time inference of this shape may still be in flight
+    // the commit-time inference of this shape may still be in flight, or
+    // waiting in the open batch (then launched now)
     harvest(*m, /*wait=*/false);
     it = m->cache.find(key);
-    bool in_flight = false;
+    bool pending = false;
     for (const Model::Pending &p : m->pending) {
-      in_flight |= p.keys[0] == key || p.keys[1] == key;
+      pending |= std::find(p.keys.begin(), p.keys.end(), key) != p.keys.end();
     }
-    if (it == m->cache.end() && in_flight) {
+    if (it == m->cache.end() && pending) {
       harvest(*m, /*wait=*/true);
       it = m->cache.find(key);
     }
@@ -656,11 +714,13 @@ int model_ranked(const Geom &g, const char *strided, const char *packed,
 }
 
 // Commit-time evaluation: the type at incount 1 with 256-B aligned buffers,
-// both directions, in one inference launch whose picks seed the shape cache.
-// The launch is not waited for: its inputs and picks sit in a pinned slot
-// until the first lookup of the shape (or a later commit) finds its event
+// both directions, in an inference launch whose picks seed the shape cache.
+// Nothing is waited for: the inputs and picks sit in a pinned slot until the
+// first lookup of the shape (or a later commit) finds its batch's event
 // fired, so MPI_Type_commit does not wait for the GPU -- nor for copies
-// already queued on the library stream.
+// already queued on the library stream.  Commits in quick succession share
+// launches (the open batch), so a burst of commits is not paced by the
+// inference kernel.
 void model_precommit(const tempi_strided_block &sb) {
   Model *m = loaded_model();
   if (!m || !have_device() || ensure_runtime()) {
@@ -678,46 +738,40 @@ void model_precommit(const tempi_strided_block &sb) {
     return;
   }
   harvest(*m, /*wait=*/false);
-  if (m->free_slots.empty()) {
-    // every slot in flight: wait for the oldest
-    const Model::Pending oldest = m->pending.front();
-    (void)hipEventSynchronize(m->ev[oldest.slot]);
-    harvest(*m, /*wait=*/false);
+  if (m->pending.empty() || m->pending.back().launched) {
+    // open a batch
+    if (m->free_slots.empty()) {
+      // every slot holds a launched batch: wait for the oldest
+      (void)hipEventSynchronize(m->ev[m->pending.front().slot]);
+      harvest(*m, /*wait=*/false);
+    }
+    if (m->free_slots.empty()) {
+      return;
+    }
+    Model::Pending b;
+    b.slot = m->free_slots.back();
+    m->free_slots.pop_back();
+    b.keys.reserve(kSlotVecs);
+    m->pending.push_back(std::move(b));
   }
-  if (m->free_slots.empty()) {
-    return;
-  }
-  const int slot = m->free_slots.back();
-  char *h = m->h_slots + slot * kSlotBytes;
-  void *d = nullptr;
-  if (!device_address(h, &d)) {
-    return;
-  }
-  float *hf = reinterpret_cast<float *>(h);
+  // this type's two feature vectors (pack, unpack) and their picks, written
+  // before the batch's launch (whose doorbell releases them to the device)
+  Model::Pending &b = m->pending.back();
+  char *h = m->h_slots + b.slot * kSlotBytes;
+  float *hf = reinterpret_cast<float *>(h) + (size_t)2 * b.n * kIn;
   copy_features(g, base, pk, 0, hf);
   copy_features(g, base, pk, 1, hf + kIn);
-  int *hb = reinterpret_cast<int *>(hf + 2 * kIn);
+  int *hb = reinterpret_cast<int *>(h + kSlotPicks) + 2 * b.n;
   hb[0] = hb[1] = -1;
-  // the kernel reads the features from and writes the picks to the pinned
-  // slot through its device mapping: one launch, no copies
-  const float *df = static_cast<const float *>(d);
-  int *db = reinterpret_cast<int *>(static_cast<char *>(d) + 2 * kIn * 4);
-  hipStream_t s = library_stream();
-  hipLaunchKernelGGL(k_mlp_infer, dim3(1), dim3(kThreads), 0, s, m->d_buf,
-                     m->d_buf + kNParams, df, 2, nullptr, db);
-  if (hipSuccess != hipGetLastError() ||
-      hipSuccess != hipEventRecord(m->ev[slot], s)) {
-    (void)hipGetLastError();
-    (void)hipStreamSynchronize(s); // the slot's buffers may still be in use
-    return;
+  b.keys.push_back(shape_key_of(g, base, pk, 0));
+  b.keys.push_back(shape_key_of(g, base, pk, 1));
+  ++b.n;
+  // launch now when no inference of ours is running (the open batch is the
+  // only one pending) or when the batch is full; else the next commit or
+  // lookup launches it
+  if (m->pending.size() == 1 || b.n == kSlotTypes) {
+    launch_open(*m);
   }
-  ++counters().model_inferences;
-  m->free_slots.pop_back();
-  Model::Pending p;
-  p.slot = slot;
-  p.keys[0] = shape_key_of(g, base, pk, 0);
-  p.keys[1] = shape_key_of(g, base, pk, 1);
-  m->pending.push_back(std::move(p));
 }
 
 // at MPI_Finalize: wait for commit-time inferences still in flight

@@ -124,3 +124,58 @@ def test_commit_time_inference_drives_mpi_pack(mpi, tmp_path):
         mpi.free(h)
     finally:
         L.tempi_mlp_load(None)
+
+
+def installed_model():
+    """(params, mean, 1/std) of tempi_amd/selector_mlp.txt."""
+    import os
+
+    path = os.path.join(os.path.dirname(ta.__file__), "selector_mlp.txt")
+    toks = open(path).read().split()
+    i = toks.index("norm")
+    norm = np.array(toks[i + 1:i + 33], float)
+    j = toks.index("params")
+    return np.array(toks[j + 1:], float), norm[:16], norm[16:]
+
+
+def test_commits_in_a_burst_share_inference_launches(mpi):
+    """Forty types committed back to back: a commit that finds an inference
+    of the library's still running joins the open batch, so there are fewer
+    launches than commits.  Every type's two picks then come from the shape
+    cache (no inference at lookup) and are, to float rounding, the argmin of
+    the CPU reference network on the same features -- a pick stored for the
+    wrong type of a batch would not be."""
+    L = ta.lib()
+    assert L.tempi_mlp_load(None) == 0
+    specs = [("vector", 100 + 7 * i, 1 + i % 5, 9 + i, tc.BYTE) for i in range(40)]
+    before = ta.counters()["model_inferences"]
+    hs = [mpi.type_from_spec(s) for s in specs]
+    launches = ta.counters()["model_inferences"] - before
+    assert 1 <= launches < len(specs), launches
+    params, mean, inv = installed_model()
+    src = ta.DeviceBuffer(1 << 20)  # 256-B aligned, as the commit assumed
+    dst = ta.DeviceBuffer(1 << 20)
+    nf = L.tempi_mlp_num_features()
+    f = (ctypes.c_float * nf)()
+    picks = []
+    for h in hs:
+        sb, _ = mpi.strided_block(h)
+        for d in (ta.DIR_PACK, ta.DIR_UNPACK):
+            picks.append((sb, d, ta.select_variant(sb, 1, src.addr(), dst.addr(), d,
+                                                   ta.SELECT_MODEL)))
+    after = ta.counters()["model_inferences"]
+    # the lookups waited for the batches still running, launched the open
+    # one (if there was one), and read every pick from the cache: a lookup
+    # the cache missed would have run an inference of its own
+    assert after - before - launches <= 1, (launches, after - before)
+    for sb, d, v in picks:
+        assert L.tempi_mlp_features(ctypes.byref(sb), 1, src.addr(), dst.addr(), d, f) == 0
+        x = (np.frombuffer(f, np.float32) - mean) * inv
+        y, _ = mlp_ref.forward(params, x[None, :], OUT)
+        assert y[0][v] <= y[0].min() + 1e-3, (sb.as_tuple(), d, v, int(y[0].argmin()))
+    n_after = ta.counters()["model_inferences"]
+    for sb, d, v in picks[:4]:
+        assert ta.select_variant(sb, 1, src.addr(), dst.addr(), d, ta.SELECT_MODEL) == v
+    assert ta.counters()["model_inferences"] == n_after  # served from the cache
+    for h in hs:
+        mpi.free(h)
