@@ -499,17 +499,30 @@ TEMPI_EXPORT int tempi_batch_create(const tempi_batch_request *reqs, int64_t n,
   // the per-copy policy first (in small batches the learned selector's
   // picks for requests under 1 MiB ran the faces of 128^3 - 256^3 float
   // grids 5-28 % faster than the closed-form rule's), then the batch rule
-  // for batches that saturate the grid (saturated_batch_policy)
+  // for batches that saturate the grid (saturated_batch_policy).  The
+  // selector evaluates all the batch's new shapes in one inference first:
+  // one launch per shape took 0.14 s to create the 4 096-request batch.
   int err = resolve_requests(
-      reqs, n,
-      [&](const Req &q) {
-        return choose(q.g, q.ds, q.dp, q.dir, method, library_stream(),
-                      /*in_place=*/false)
-            .variant;
-      },
-      &rs, &where, &b->bytes);
+      reqs, n, [](const Req &) { return -1; }, &rs, &where, &b->bytes);
   if (err) {
     return err;
+  }
+  if (method == TEMPI_SELECT_MODEL ||
+      (method == TEMPI_SELECT_DEFAULT && model_available())) {
+    std::vector<CopyShape> shapes;
+    for (const Req &q : rs) {
+      if (q.variant < 0) {
+        shapes.push_back({q.g, q.ds, q.dp, q.dir});
+      }
+    }
+    model_prefetch(shapes);
+  }
+  for (Req &q : rs) {
+    if (q.variant < 0) {
+      q.variant = choose(q.g, q.ds, q.dp, q.dir, method, library_stream(),
+                         /*in_place=*/false)
+                      .variant;
+    }
   }
   if (method == TEMPI_SELECT_DEFAULT) {
     saturated_batch_policy(reqs, where, &rs);

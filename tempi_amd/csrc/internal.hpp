@@ -271,6 +271,17 @@ bool model_available();
 int model_variant(const Geom &g, const char *strided, const char *packed,
                   int dir, const std::string &key); // -1: no model
 void model_precommit(const tempi_strided_block &sb);
+// one copy of a set whose variants are about to be chosen
+struct CopyShape {
+  Geom g;
+  const char *strided;
+  const char *packed;
+  int dir;
+};
+// evaluates the selector once for every shape of `copies` not yet in its
+// cache (copies timed cold are left to their trials), so the per-copy
+// choices that follow are cache hits
+void model_prefetch(const std::vector<CopyShape> &copies);
 // waits for commit-time inferences still in flight (at MPI_Finalize)
 void model_finalize();
 // inputs of the selector networks (variants and staging methods)

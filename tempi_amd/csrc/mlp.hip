@@ -29,6 +29,7 @@
 #include <deque>
 #include <map>
 #include <random>
+#include <set>
 #include <string>
 
 namespace tempi {
@@ -275,7 +276,8 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Predicted relative time of every variant for n raw feature vectors, and
-// the argmin of each.
+// the argmin of each; workgroup w takes chunks w, w + grid, ... of kB
+// vectors.
 __global__ void __launch_bounds__(kThreads)
     k_mlp_infer(const float *__restrict__ params,
                 const float *__restrict__ norm, const float *__restrict__ X,
@@ -284,7 +286,7 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ Acts a;
   load_params(P, params);
   __syncthreads();
-  for (int b0 = 0; b0 < n; b0 += kB) {
+  for (int b0 = (int)blockIdx.x * kB; b0 < n; b0 += (int)gridDim.x * kB) {
     const int nb = min(kB, n - b0);
     for (int idx = threadIdx.x; idx < nb * kIn; idx += kThreads) {
       const int b = idx / kIn, i = idx % kIn;
@@ -437,6 +439,45 @@ static int model_infer(Model &m, const float *x, int n, int *best,
   TEMPI_HIP_CHECK(hipStreamSynchronize(s));
   ++counters().model_inferences;
   return TEMPI_SUCCESS;
+}
+
+// run the inference kernel on n raw feature vectors (any n), one workgroup
+// per chunk of kB, in buffers from the device pool
+static int model_infer_many(Model &m, const float *x, int n, int *best) {
+  int err = model_device_buffers(m);
+  if (err) {
+    return err;
+  }
+  const size_t xb = (size_t)n * kIn * 4;
+  char *d = static_cast<char *>(pool_acquire(false, xb + (size_t)n * 4));
+  if (!d) {
+    return set_error(TEMPI_ERR_NOMEM, "selector inputs of %d copies", n);
+  }
+  hipStream_t s = library_stream();
+  float *d_x = reinterpret_cast<float *>(d);
+  int *d_best = reinterpret_cast<int *>(d + xb);
+  const int grid = std::min((n + kB - 1) / kB, 1024);
+  err = TEMPI_SUCCESS;
+  if (hipSuccess != hipMemcpyAsync(d_x, x, xb, hipMemcpyHostToDevice, s)) {
+    err = set_error(TEMPI_ERR_HIP, "selector input upload failed");
+  } else {
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kThreads), 0, s, m.d_buf,
+                       m.d_buf + kNParams, d_x, n, nullptr, d_best);
+    if (hipSuccess != hipGetLastError() ||
+        hipSuccess != hipMemcpyAsync(best, d_best, (size_t)n * 4,
+                                     hipMemcpyDeviceToHost, s)) {
+      err = set_error(TEMPI_ERR_HIP, "selector inference failed");
+    }
+  }
+  // the buffers go back to the pool only after the stream is done with them
+  if (hipSuccess != hipStreamSynchronize(s) && !err) {
+    err = set_error(TEMPI_ERR_HIP, "selector inference failed");
+  }
+  pool_release(false, d);
+  if (!err) {
+    ++counters().model_inferences;
+  }
+  return err;
 }
 
 // Launch the open batch (the newest pending one, if not launched yet): one
@@ -771,6 +812,42 @@ void model_precommit(const tempi_strided_block &sb) {
   // lookup launches it
   if (m->pending.size() == 1 || b.n == kSlotTypes) {
     launch_open(*m);
+  }
+}
+
+void model_prefetch(const std::vector<CopyShape> &copies) {
+  Model *m = loaded_model();
+  if (!m || copies.empty() || ensure_runtime()) {
+    return;
+  }
+  std::lock_guard<std::mutex> lk(m->mu);
+  harvest(*m, /*wait=*/false);
+  std::vector<float> x;
+  std::vector<std::string> keys;
+  std::set<std::string> seen;
+  for (const CopyShape &c : copies) {
+    if (times_cold(c.g)) {
+      continue;
+    }
+    std::string key = shape_key_of(c.g, c.strided, c.packed, c.dir);
+    if (m->cache.count(key) || !seen.insert(key).second) {
+      continue;
+    }
+    x.resize(x.size() + kIn);
+    copy_features(c.g, c.strided, c.packed, c.dir, x.data() + x.size() - kIn);
+    keys.push_back(std::move(key));
+  }
+  if (keys.empty()) {
+    return;
+  }
+  std::vector<int> best(keys.size(), -1);
+  if (model_infer_many(*m, x.data(), (int)keys.size(), best.data())) {
+    return; // the per-copy choices evaluate these shapes one by one
+  }
+  for (size_t i = 0; i < keys.size(); ++i) {
+    if (best[i] >= 0 && best[i] < kOut) {
+      m->cache[keys[i]] = best[i];
+    }
   }
 }
 

@@ -179,3 +179,35 @@ def test_commits_in_a_burst_share_inference_launches(mpi):
     assert ta.counters()["model_inferences"] == n_after  # served from the cache
     for h in hs:
         mpi.free(h)
+
+
+def test_batch_creation_evaluates_new_shapes_in_one_inference():
+    """A batch of 200 requests of shapes the selector has not seen is
+    created with one inference over all of them (one launch per shape made
+    large heterogeneous batches slow to create); each request's variant is,
+    to float rounding, the CPU reference network's argmin on its features."""
+    L = ta.lib()
+    assert L.tempi_mlp_load(None) == 0
+    params, mean, inv = installed_model()
+    src = ta.DeviceBuffer(64 << 20)
+    dst = ta.DeviceBuffer(64 << 20)
+    reqs, off = [], 0
+    for i in range(200):
+        rows, bl = 50 + 13 * i, 1 + (7 * i) % 97
+        sb = ta.StridedBlock.make(i % 16, (rows - 1) * (bl + 3 + i) + bl + 16, [bl, rows],
+                                  [1, bl + 3 + i])
+        assert i % 16 + sb.extent <= src.nbytes and off + sb.size() <= dst.nbytes
+        reqs.append((sb, src.addr(), dst.addr(off), 1, ta.DIR_PACK if i % 3 else ta.DIR_UNPACK))
+        off += sb.size()
+    before = ta.counters()["model_inferences"]
+    b = ta.Batch(reqs, method=ta.SELECT_MODEL)
+    assert ta.counters()["model_inferences"] - before == 1
+    nf = L.tempi_mlp_num_features()
+    f = (ctypes.c_float * nf)()
+    for i, (sb, s, p, count, d) in enumerate(reqs):
+        v = b.variant(i)
+        assert L.tempi_mlp_features(ctypes.byref(sb), count, s, p, d, f) == 0
+        x = (np.frombuffer(f, np.float32) - mean) * inv
+        y, _ = mlp_ref.forward(params, x[None, :], OUT)
+        assert y[0][v] <= y[0].min() + 1e-3, (i, v, int(y[0].argmin()))
+    b.destroy()
