@@ -14,8 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
-#include <map>
-#include <sstream>
+#include <unordered_map>
 
 namespace tempi {
 
@@ -108,20 +107,29 @@ int flush_caches_on(int64_t bytes, hipStream_t s) {
 
 namespace {
 std::mutex g_tune_mu;
-std::map<std::string, Choice> g_tune_cache;
-std::map<std::string, Choice> g_verify_cache; // default policy, large copies
+std::unordered_map<std::string, Choice> g_tune_cache;
+std::unordered_map<std::string, Choice> g_verify_cache; // default policy, large copies
 } // namespace
 
+// The key is the raw bytes of the facts (direction, row length, row
+// dimensions, the alignments modulo 16 of both sides), not text: it is
+// built for every copy a caller makes and never stored, and formatting it
+// with a stream took about ten times as long as the lookup it served.
 std::string shape_key_of(const Geom &g, const char *strided,
                          const char *packed, int dir) {
-  std::ostringstream os;
-  os << dir << ':' << g.L << ':' << g.nrd;
+  int64_t w[4 + 2 * kMaxRowDims];
+  int n = 0;
+  w[n++] = dir;
+  w[n++] = g.L;
+  w[n++] = g.nrd;
   for (int k = 0; k < g.nrd; ++k) {
-    os << ':' << g.cnt[k] << 'x' << g.str[k];
+    w[n++] = g.cnt[k];
+    w[n++] = g.str[k];
   }
-  os << ":a" << ((uintptr_t)(strided + g.start) & 15) << '/'
-     << ((uintptr_t)packed & 15);
-  return os.str();
+  w[n++] = (int64_t)(((uintptr_t)(strided + g.start) & 15) |
+                     (((uintptr_t)packed & 15) << 8));
+  return std::string(reinterpret_cast<const char *>(w),
+                     (size_t)n * sizeof(int64_t));
 }
 
 bool times_cold(const Geom &g) { return g.L * g.rows >= kColdBytes; }

@@ -31,6 +31,7 @@
 #include <random>
 #include <set>
 #include <string>
+#include <unordered_map>
 
 namespace tempi {
 
@@ -338,7 +339,7 @@ struct Model {
   std::vector<float> params;   // kNParams
   std::vector<float> norm;     // mean[kIn], 1/std[kIn]
   float *d_buf = nullptr;      // params, norm, x, pred, best
-  std::map<std::string, int> cache;
+  std::unordered_map<std::string, int> cache;
   std::string path;
   struct Pending {
     int slot;
