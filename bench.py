@@ -711,7 +711,9 @@ def bench_halos(ta, m, tc, orc, stream):
     hosts = [orc.splitmix_bytes(sub_bytes, 4000 + s) for s in range(64)]
     reqs, total, keep = halo_requests(ta, m, tc, hosts)
     packed = keep[-1]
+    t_create = time.perf_counter()
     b = ta.Batch(reqs)
+    create_ms = (time.perf_counter() - t_create) * 1e3
     nreq = len(reqs)
     for _ in range(5):
         b.launch(stream)
@@ -763,6 +765,7 @@ def bench_halos(ta, m, tc, orc, stream):
     wall_pa /= 3
     res = {
         "requests": nreq, "packed_bytes": total,
+        "create_ms": round(create_ms, 2),
         "batched_packs_per_s": round(nreq * reps / wall_b, 1),
         "graph_packs_per_s": round(nreq * reps / wall_g, 1),
         "per_call_sync_packs_per_s": round(nreq / wall_pc, 1),
@@ -850,7 +853,9 @@ def bench_random_requests(ta, tc, orc, stream, nreq=4096, seed=4):
     for sb in reqs_geo:
         reqs.append((sb, src.addr(), packed.addr(off), 1, ta.DIR_PACK))
         off += sb.size()
+    t_create = time.perf_counter()
     b = ta.Batch(reqs)
+    create_ms = (time.perf_counter() - t_create) * 1e3
     b.launch(stream)
     reps = 5
     wall_b, dev_b = timed_loop(ta, lambda: b.launch(stream), reps, stream)
@@ -884,6 +889,7 @@ def bench_random_requests(ta, tc, orc, stream, nreq=4096, seed=4):
     per_call()
     wall_pc = time.perf_counter() - t0
     res = {"requests": nreq, "packed_bytes": total,
+           "create_ms": round(create_ms, 2),
            "batched_packs_per_s": round(nreq * reps / wall_b, 1),
            "per_call_sync_packs_per_s": round(nreq / wall_pc, 1),
            "batch_kernel_hbm_gbs": round(2 * total / (dev_b / reps) / 1e9, 1),
