@@ -211,3 +211,40 @@ def test_batch_creation_evaluates_new_shapes_in_one_inference():
         y, _ = mlp_ref.forward(params, x[None, :], OUT)
         assert y[0][v] <= y[0].min() + 1e-3, (i, v, int(y[0].argmin()))
     b.destroy()
+
+
+def test_weights_loaded_while_commit_batches_are_pending_replace_their_picks(mpi, tmp_path):
+    """Twenty types committed back to back under the installed weights leave
+    inferences running and an open batch; loading other weights right after
+    drops those picks (the open batch is never launched), so every lookup
+    answers with the new network's argmin."""
+    L = ta.lib()
+    assert L.tempi_mlp_load(None) == 0
+    params = mlp_ref.init_params(OUT, 77)
+    mean = np.zeros(16, np.float32)
+    inv = np.full(16, 0.25, np.float32)
+    path = str(tmp_path / "other.txt")
+    assert L.tempi_mlp_save(path.encode(), params.ctypes.data,
+                            np.concatenate([mean, inv]).ctypes.data) == 0
+    specs = [("vector", 300 + 11 * i, 2 + i % 3, 17 + i, tc.BYTE) for i in range(20)]
+    hs = [mpi.type_from_spec(s) for s in specs]
+    try:
+        assert L.tempi_mlp_load(path.encode()) == 0
+        src = ta.DeviceBuffer(1 << 20)
+        dst = ta.DeviceBuffer(1 << 20)
+        nf = L.tempi_mlp_num_features()
+        f = (ctypes.c_float * nf)()
+        for h in hs:
+            sb, _ = mpi.strided_block(h)
+            for d in (ta.DIR_PACK, ta.DIR_UNPACK):
+                v = ta.select_variant(sb, 1, src.addr(), dst.addr(), d, ta.SELECT_MODEL)
+                assert L.tempi_mlp_features(ctypes.byref(sb), 1, src.addr(), dst.addr(), d,
+                                            f) == 0
+                x = (np.frombuffer(f, np.float32) - mean) * inv
+                y, _ = mlp_ref.forward(params, x[None, :], OUT)
+                assert y[0][v] <= y[0].min() + 1e-3, (sb.as_tuple(), d, v,
+                                                      int(y[0].argmin()))
+    finally:
+        L.tempi_mlp_load(None)
+        for h in hs:
+            mpi.free(h)
