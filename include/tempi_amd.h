@@ -267,7 +267,10 @@ int tempi_launch_latency_us(int reps, float *host_us, float *device_us);
    device-resident descriptor table; one launch runs a persistent kernel that
    drains all of them (a batch of at most 8 requests passes its tile index
    in the kernel arguments instead of a device tile table).  tempi_batch_graph_launch replays the same launch from
-   a captured hipGraph.  Requests in one batch must be independent (no
+   a captured hipGraph; it is slower than tempi_batch_launch on ROCm 7.2,
+   whose hipGraphLaunch costs about 17 us of host time against about 2.7 us
+   for a kernel launch, and is for callers whose own step is a graph.
+   Requests in one batch must be independent (no
    request writes bytes another reads or writes).  Nothing is synchronized:
    wait on the stream.  `method` (TEMPI_SELECT_*) picks the variant of each
    request that leaves it to the library (variant -1), as for a single copy
