@@ -468,9 +468,15 @@ int tempi_perf_save(const char *path);
 /* Fill the empty entries this GPU can measure (reference: measure_performance,
    src/internal/measure_system.cu:381-609): bit 0 kernel launch, bit 1 the
    d2h/h2d curves, bit 2 the device pack/unpack tables, bit 3 the mapped-host
-   (ONE_SHOT) tables.  Each point takes at least 7 samples and at most
-   `seconds_per_point` (<= 0: 0.05 s) or 500 samples; its time is their
-   trimean (src/internal/statistics.cpp:30-39).
+   (ONE_SHOT) tables.  Each point runs trials of at least 7 samples and at
+   most `seconds_per_point` (<= 0: 0.05 s) or 500 samples.  After each
+   trial, tempi_iid_test decides whether its samples are independent and
+   identically distributed.  Trials repeat until the test passes or
+   $TEMPI_MEASURE_TRIALS trials have run (default 10, the reference's
+   RunConfig::maxTrials, include/benchmark.hpp:41-46).  The point's time
+   is the last trial's trimean (src/internal/statistics.cpp:30-39), and its
+   `iid` flag in perf.json is the test's verdict
+   (src/internal/benchmark.cpp:44-83).
    Bit 4 (TEMPI_MEASURE_PINGPONG) is collective over MPI_COMM_WORLD and needs
    at least two ranks: ranks 0 and 1 bounce host messages of 1 B - 8 MiB
    through the MPI library (reference: the CPU-CPU ping-pong of
@@ -481,6 +487,23 @@ int tempi_perf_save(const char *path);
    GPU-aware MPI library and stay imported. */
 #define TEMPI_MEASURE_PINGPONG 16
 int tempi_perf_measure(int what, double seconds_per_point);
+/* The permutation test of NIST SP 800-90B section 5.1 on `n` samples
+   (2 <= n <= 65536, all finite), with the eight statistics the reference
+   uses (reference: sp_800_90B, src/internal/iid.cpp:184-245; test/iid.cpp).
+   Statistics, in order: excursion, number of directional runs, longest
+   directional run, number of increases or decreases, number of runs about
+   the median, longest run about the median, average collision, maximum
+   collision.  tempi_iid_statistics writes the eight statistics of the
+   samples to stats[8].  tempi_iid_test shuffles the samples 10 000 times;
+   shuffle k uses a splitmix64 stream seeded with
+   seed + k * 0x9E3779B97F4A7C15.  It writes counts[2 s] = C0, the shuffles
+   whose statistic s exceeds the samples', and counts[2 s + 1] = C1, those
+   equal to it (counts may be NULL).  It returns 1 when no statistic has
+   C0 + C1 <= 5 or C0 >= 9995 (independent), 0 when one does, and
+   TEMPI_ERR_ARG for bad input. */
+int tempi_iid_statistics(const double *samples, int64_t n, double *stats);
+int tempi_iid_test(const double *samples, int64_t n, uint64_t seed,
+                   int64_t *counts);
 /* Modelled time of sending `bytes` of non-contiguous data in blocks of
    `block_length` by `method`; TEMPI_ERR_UNSUPPORTED when an input is missing.
    reference: DeviceND/OneshotND/StagedND::model, src/internal/sender.cpp:

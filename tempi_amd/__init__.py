@@ -242,6 +242,8 @@ def _prototypes(L):
         "tempi_perf_load": (_I, [ctypes.c_char_p]),
         "tempi_perf_save": (_I, [ctypes.c_char_p]),
         "tempi_perf_measure": (_I, [_I, ctypes.c_double]),
+        "tempi_iid_statistics": (_I, [_P, _I64, _P]),
+        "tempi_iid_test": (_I, [_P, _I64, ctypes.c_uint64, _P]),
         "tempi_perf_model": (_I, [_I, _I, _I64, _I64, ctypes.POINTER(ctypes.c_double)]),
         "tempi_choose_method": (_I, [_I, _I64, _I64]),
         "tempi_p2p_method": (_I, [_I, _I64, _I64, _I64]),

@@ -30,6 +30,7 @@ SOURCES = [
     "batch.hip",
     "mlp.hip",
     "perf.hip",
+    "iid.cpp",
     "pool.cpp",
     "p2p.cpp",
     "neighbor.cpp",

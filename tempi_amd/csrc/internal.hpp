@@ -341,6 +341,15 @@ uint64_t perf_generation();
 double perf_model_seconds(int method, bool colocated, int64_t bytes,
                           int64_t block_length);
 
+// NIST SP 800-90B section 5.1 permutation test of measured samples
+// (iid.cpp): the eight statistics of `n` samples into t[kIidStats], and the
+// test itself -- 1 independent, 0 not, a TEMPI_ERR_* code for bad input;
+// `counts` (if not null) receives C0 and C1 of each statistic.
+constexpr int kIidStats = 8;
+constexpr int64_t kIidMaxSamples = 1 << 16;
+int iid_statistics(const double *samples, int64_t n, double *t);
+int iid_test(const double *samples, int64_t n, uint64_t seed, int64_t *counts);
+
 // the per-type record cache (interpose.cpp); predefined types are lowered
 // on their first lookup
 bool type_lookup(MPI_Datatype t, Canon *out);

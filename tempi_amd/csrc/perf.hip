@@ -534,24 +534,49 @@ double trimean(std::vector<double> x) {
   return (x[q] + 2 * x[2 * q] + x[3 * q]) / 4;
 }
 
-// One point: a warm-up sample, then at least 7 and at most 500 samples or
-// `max_s` seconds.  The reference repeats whole trials until an SP 800-90B
-// IID test passes; that test is not run here, so iid is recorded false.
-template <class F> int sample_point(F &&iter, double max_s, IidTime *out) {
+// How each point is sampled: trials of at least 7 and at most 500 samples
+// or `max_s` seconds, repeated until the samples pass the SP 800-90B
+// permutation test (iid.cpp) or `trials` trials have run (reference:
+// Benchmark::run, src/internal/benchmark.cpp:44-83).
+struct Sampling {
+  double max_s;
+  int trials;
+};
+
+// seeds of successive IID tests: a fixed sequence, so that a run's verdicts
+// depend only on its samples
+static std::atomic<uint64_t> g_iid_calls{0};
+
+// One point: a warm-up sample, then trials as above.  The time is the last
+// trial's trimean; `iid` records whether that trial passed the test.
+template <class F> int sample_point(F &&iter, const Sampling &sp, IidTime *out) {
   double t = 0;
   int err = iter(&t);
   std::vector<double> xs;
-  const auto t0 = Clock::now();
-  while (!err && (xs.size() < 7 || (seconds_since(t0) < max_s && xs.size() < 500))) {
-    err = iter(&t);
-    xs.push_back(t);
+  for (int trial = 1; !err; ++trial) {
+    xs.clear();
+    const auto t0 = Clock::now();
+    while (!err && (xs.size() < 7 ||
+                    (seconds_since(t0) < sp.max_s && xs.size() < 500))) {
+      err = iter(&t);
+      xs.push_back(t);
+    }
+    if (err) {
+      break;
+    }
+    const uint64_t seed = 0x243F6A8885A308D3ull +
+                          g_iid_calls.fetch_add(1) * 0x9E3779B97F4A7C15ull;
+    const int iid = iid_test(xs.data(), (int64_t)xs.size(), seed, nullptr);
+    if (iid < 0) {
+      return iid;
+    }
+    if (iid == 1 || trial >= sp.trials) {
+      out->time = trimean(xs);
+      out->iid = iid == 1;
+      return TEMPI_SUCCESS;
+    }
   }
-  if (err) {
-    return err;
-  }
-  out->time = trimean(xs);
-  out->iid = false;
-  return TEMPI_SUCCESS;
+  return err;
 }
 
 __global__ void k_perf_empty(int *p) {
@@ -577,7 +602,7 @@ struct HostBuf {
   }
 };
 
-int measure_launch(double max_s, double *out) {
+int measure_launch(const Sampling &sp, double *out) {
   hipStream_t s = library_stream();
   IidTime r;
   int err = sample_point(
@@ -591,7 +616,7 @@ int measure_launch(double max_s, double *out) {
         TEMPI_HIP_CHECK(hipStreamSynchronize(s));
         return TEMPI_SUCCESS;
       },
-      max_s, &r);
+      sp, &r);
   if (!err) {
     *out = r.time;
   }
@@ -600,7 +625,7 @@ int measure_launch(double max_s, double *out) {
 
 // d2h / h2d of 1 B .. 8 MiB between device memory and pinned host memory,
 // each sample repeated to last about 200 us (measure_system.cu:97-170)
-int measure_copies(int which, double max_s, std::vector<IidTime> *out) {
+int measure_copies(int which, const Sampling &sp, std::vector<IidTime> *out) {
   const size_t maxb = (size_t)1 << (TEMPI_PERF_CURVE_LEN - 1);
   DevBuf d;
   HostBuf h;
@@ -635,7 +660,7 @@ int measure_copies(int which, double max_s, std::vector<IidTime> *out) {
     }
     reps = std::max(1, (int)(200e-6 / std::max(t1, 1e-9)));
     IidTime r;
-    err = sample_point(once, max_s, &r);
+    err = sample_point(once, sp, &r);
     if (err) {
       return err;
     }
@@ -711,7 +736,7 @@ int measure_pingpong(int *curve, std::vector<IidTime> *out) {
 // a synchronous pack or unpack of 2^(2i+6) bytes in blocks of
 // min(bytes, 2^j) bytes at a 512-B stride, as MPI_Pack runs it (the variant
 // the library's selector picks, then a stream synchronization)
-int measure_table(int table, double max_s, Table *out) {
+int measure_table(int table, const Sampling &sampling, Table *out) {
   TrialScope trial; // the table's copies are measurement, not the caller's
   const int64_t max_bytes = 1ll << (2 * (TEMPI_PERF_TABLE_DIM - 1) + 6);
   const int64_t stride = 512;
@@ -780,7 +805,7 @@ int measure_table(int table, double max_s, Table *out) {
             *tm = seconds_since(t0);
             return TEMPI_SUCCESS;
           },
-          max_s, &r);
+          sampling, &r);
       if (err) {
         return err;
       }
@@ -1000,7 +1025,14 @@ TEMPI_EXPORT int tempi_perf_measure(int what, double seconds_per_point) {
   if ((what & 0xF) && (err = ensure_runtime())) {
     return err;
   }
-  const double max_s = seconds_per_point > 0 ? seconds_per_point : 0.05;
+  Sampling sp{seconds_per_point > 0 ? seconds_per_point : 0.05, 10};
+  if (const char *e = std::getenv("TEMPI_MEASURE_TRIALS")) {
+    int v = 0;
+    if (!parse_number(e, e + std::strlen(e), &v) || v < 1) {
+      return set_error(TEMPI_ERR_ARG, "TEMPI_MEASURE_TRIALS=%s", e);
+    }
+    sp.trials = v;
+  }
   // measure outside the lock, fill only what is still empty
   Performance cur;
   {
@@ -1021,7 +1053,7 @@ TEMPI_EXPORT int tempi_perf_measure(int what, double seconds_per_point) {
   }
   if ((what & 1) && cur.kernel_launch == 0) {
     double t = 0;
-    if ((err = measure_launch(max_s, &t))) {
+    if ((err = measure_launch(sp, &t))) {
       return err;
     }
     std::lock_guard<std::mutex> lk(g_perf_mu);
@@ -1033,7 +1065,7 @@ TEMPI_EXPORT int tempi_perf_measure(int what, double seconds_per_point) {
         continue;
       }
       std::vector<IidTime> v;
-      if ((err = measure_copies(c, max_s, &v))) {
+      if ((err = measure_copies(c, sp, &v))) {
         return err;
       }
       std::lock_guard<std::mutex> lk(g_perf_mu);
@@ -1046,7 +1078,7 @@ TEMPI_EXPORT int tempi_perf_measure(int what, double seconds_per_point) {
       continue;
     }
     Table tb;
-    if ((err = measure_table(t, max_s, &tb))) {
+    if ((err = measure_table(t, sp, &tb))) {
       return err;
     }
     std::lock_guard<std::mutex> lk(g_perf_mu);

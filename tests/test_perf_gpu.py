@@ -1,7 +1,10 @@
 """The system-performance tables measured on the GPU with the library's own
 pack kernels (reference: measure_performance,
 src/internal/measure_system.cu:381-609), and the method choice on them."""
+import json
 import math
+import os
+import time
 
 import numpy as np
 import pytest
@@ -17,8 +20,14 @@ pytestmark = pytest.mark.gpu
 def measured():
     assert ta.device_count() > 0
     tsys.clear()
-    print("measuring system tables (5 ms per point)", flush=True)
-    tsys.measure_performance(tsys.MEASURE_ALL, 0.005)
+    print("measuring system tables (5 ms per trial, at most 3 trials a point)", flush=True)
+    os.environ["TEMPI_MEASURE_TRIALS"] = "3"
+    t0 = time.perf_counter()
+    try:
+        tsys.measure_performance(tsys.MEASURE_ALL, 0.005)
+    finally:
+        del os.environ["TEMPI_MEASURE_TRIALS"]
+    print(f"measured in {time.perf_counter() - t0:.1f} s", flush=True)
     snap = {
         "launch": tsys.kernel_launch(),
         "curves": [tsys.get_curve(c) for c in range(6)],
@@ -50,6 +59,23 @@ def test_tables_have_the_expected_shape_of_costs(measured):
     assert dev[8][8] < dev[8][0]
     # 8 MiB host copies take longer than 1-B ones
     assert measured["curves"][tsys.D2H][23] > measured["curves"][tsys.D2H][0]
+
+
+def test_every_entry_records_the_iid_verdict(measured, tmp_path):
+    """Each measured point carries the permutation test's verdict on its
+    last trial (reference: Benchmark::run, src/internal/benchmark.cpp:44-83):
+    a boolean in perf.json.  Timing samples of a GPU are not always
+    independent, but most points pass within three trials."""
+    path = str(tmp_path / "perf.json")
+    tsys.export_performance(path)
+    perf = json.load(open(path))
+    flags = [e["iid"] for k in ("d2h", "h2d") for e in perf[k]]
+    flags += [e["iid"] for k in tsys.TABLE_KEYS for row in perf[k] for e in row]
+    assert len(flags) == 2 * tsys.CURVE_LEN + 4 * 81
+    assert all(isinstance(f, bool) for f in flags)
+    share = sum(flags) / len(flags)
+    print(f"entries that passed the IID test: {share:.2f}", flush=True)
+    assert share >= 0.5
 
 
 def test_measure_fills_only_empty_entries(measured):
