@@ -158,3 +158,20 @@ def test_split_copies_stay_exact():
     sb = SB(11, 10000, [9, 5, 7], [1, 13, 100])
     for variant in range(ta.num_variants()):
         check_copy(sb, 3, 5, 9, ta.DIR_UNPACK, variant)
+
+
+def test_every_tiny_copy_every_variant():
+    """Every tiny 2D copy of tests/typecatalog.py (rows of 1-16 B, 1-4 rows,
+    strides up to 16 B, the strided side at each offset 0-15, one and two
+    objects), with the packed side at every offset modulo 16, under every
+    variant in both directions: stores exactly the type map and the packed
+    stream, loads inside them -- the host-side check of the copies that
+    tests/test_tiny_exhaustive_gpu.py runs on the GPU."""
+    n = 0
+    for k, (start, ext, counts, strides, off, inc) in enumerate(tc.tiny_copies()):
+        sb = SB(start, ext, counts, strides)
+        for variant in range(ta.num_variants()):
+            for direction in (ta.DIR_PACK, ta.DIR_UNPACK):
+                check_copy(sb, inc, off, (k + variant) % 16, direction, variant)
+                n += 1
+    assert n == len(tc.tiny_copies()) * ta.num_variants() * 2

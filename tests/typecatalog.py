@@ -266,3 +266,25 @@ def random_requests(seed: int = 4, nreq: int = 4096):
         out.append((rng.randint(0, 3) * 4, span + 16, counts, strides))
         span_max = max(span_max, span + 64)
     return out, span_max
+
+
+# Every tiny 2D byte copy: rows of 1-16 B, 1-4 rows at strides from the row
+# length up to 16 B, the strided side at each of the 16 byte offsets below a
+# 16-B boundary, one object; and each shape again as two objects (extent =
+# rows x stride + 4) at offset 5.  The largest strided span is
+# 15 + 3 * 16 + 16 = 79 bytes from the buffer start (two objects: 5 + 68 + 64
+# = 137).
+TINY_SPAN = 160
+
+
+def tiny_copies():
+    """[(start, extent, counts, strides, offset, incount)]."""
+    out = []
+    for bl in range(1, 17):
+        for rows in range(1, 5):
+            for stride in range(bl, 17):
+                extent = rows * stride
+                for off in range(16):
+                    out.append((0, extent, [bl, rows], [1, stride], off, 1))
+                out.append((0, extent + 4, [bl, rows], [1, stride], 5, 2))
+    return out
