@@ -160,18 +160,21 @@ def test_split_copies_stay_exact():
         check_copy(sb, 3, 5, 9, ta.DIR_UNPACK, variant)
 
 
-def test_every_tiny_copy_every_variant():
-    """Every tiny 2D copy of tests/typecatalog.py (rows of 1-16 B, 1-4 rows,
-    strides up to 16 B, the strided side at each offset 0-15, one and two
-    objects), with the packed side at every offset modulo 16, under every
-    variant in both directions: stores exactly the type map and the packed
-    stream, loads inside them -- the host-side check of the copies that
-    tests/test_tiny_exhaustive_gpu.py runs on the GPU."""
+@pytest.mark.parametrize("kind", ["2d", "3d"])
+def test_every_tiny_copy_every_variant(kind):
+    """Every tiny copy of tests/typecatalog.py -- 2D: rows of 1-16 B, 1-4
+    rows, strides up to 16 B, one and two objects; 3D: 1-3 rows by 1-3
+    planes, negative plane strides among them -- with the strided side at
+    each offset 0-15 and the packed side at every offset modulo 16, under
+    every variant in both directions: stores exactly the type map and the
+    packed stream, loads inside them -- the host-side check of the copies
+    that tests/test_tiny_exhaustive_gpu.py runs on the GPU."""
+    copies = tc.tiny_copies() if kind == "2d" else tc.tiny_copies_3d()
     n = 0
-    for k, (start, ext, counts, strides, off, inc) in enumerate(tc.tiny_copies()):
+    for k, (start, ext, counts, strides, off, inc) in enumerate(copies):
         sb = SB(start, ext, counts, strides)
         for variant in range(ta.num_variants()):
             for direction in (ta.DIR_PACK, ta.DIR_UNPACK):
                 check_copy(sb, inc, off, (k + variant) % 16, direction, variant)
                 n += 1
-    assert n == len(tc.tiny_copies()) * ta.num_variants() * 2
+    assert n == len(copies) * ta.num_variants() * 2

@@ -1,6 +1,7 @@
-"""Brute force on the GPU: every tiny 2D copy (rows of 1-16 B, 1-4 rows at
-strides up to 16 B, the strided side at each byte offset 0-15, one and two
-objects; tests/typecatalog.py) packed and unpacked by every kernel variant,
+"""Brute force on the GPU: every tiny copy of tests/typecatalog.py -- 2D:
+rows of 1-16 B, 1-4 rows at strides up to 16 B, one and two objects; 3D:
+1-3 rows by 1-3 planes, negative plane strides among them; the strided side
+at each byte offset 0-15 -- packed and unpacked by every kernel variant,
 through the single-copy kernel and through the batch kernel, against the
 oracle's gather and scatter.  Packed slices sit at every alignment with
 guard bytes between them; each unpack writes its own 0xAB-filled region, so
@@ -15,13 +16,14 @@ from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
-REGION = 192  # destination bytes per unpack (>= tc.TINY_SPAN)
-
-
-@pytest.fixture(scope="module")
-def tiny():
-    copies = tc.tiny_copies()
-    host = orc.splitmix_bytes(tc.TINY_SPAN, 31)
+@pytest.fixture(scope="module", params=["2d", "3d"])
+def tiny(request):
+    if request.param == "2d":
+        copies, span = tc.tiny_copies(), tc.TINY_SPAN
+    else:
+        copies, span = tc.tiny_copies_3d(), tc.TINY3_SPAN
+    region = span  # destination bytes per unpack
+    host = orc.splitmix_bytes(span, 31)
     sbs, exp, offs = [], [], []
     pos = 0
     for k, (start, ext, counts, strides, off, inc) in enumerate(copies):
@@ -34,17 +36,17 @@ def tiny():
     packed = np.full(pos + 64, 0x5A, np.uint8)
     for e, o in zip(exp, offs):
         packed[o:o + e.size] = e
-    image = np.full(len(copies) * REGION, 0xAB, np.uint8)
+    image = np.full(len(copies) * region, 0xAB, np.uint8)
     for k, ((_, _, _, _, off, inc), sb, e) in enumerate(zip(copies, sbs, exp)):
-        orc.sb_unpack(orc.sb_desc(*sb.as_tuple()), e, image[k * REGION:(k + 1) * REGION],
+        orc.sb_unpack(orc.sb_desc(*sb.as_tuple()), e, image[k * region:(k + 1) * region],
                       off, inc)
-    src = ta.DeviceBuffer(tc.TINY_SPAN)
+    src = ta.DeviceBuffer(span)
     src.upload(host)
     psrc = ta.DeviceBuffer(packed.size)
     psrc.upload(packed)
-    print(f"{len(copies)} tiny copies, {pos} packed bytes", flush=True)
+    print(f"{len(copies)} tiny {request.param} copies, {pos} packed bytes", flush=True)
     return dict(copies=copies, sbs=sbs, exp=exp, offs=offs, packed=packed, image=image,
-                src=src, psrc=psrc)
+                src=src, psrc=psrc, region=region)
 
 
 def check(t, pk, dst, what):
@@ -53,20 +55,20 @@ def check(t, pk, dst, what):
     assert bad.size == 0, (what, "pack", int(bad[0]), len(bad))
     got = dst.download()
     bad = np.nonzero(got != t["image"])[0]
-    assert bad.size == 0, (what, "unpack", t["copies"][int(bad[0]) // REGION], len(bad))
+    assert bad.size == 0, (what, "unpack", t["copies"][int(bad[0]) // t["region"]], len(bad))
 
 
 @pytest.mark.parametrize("variant", range(19))
 def test_every_tiny_copy_single_copy_kernel(tiny, variant):
-    t = tiny
+    t, rg = tiny, tiny["region"]
     pk = ta.DeviceBuffer(t["packed"].size)
     pk.fill(0x5A)
-    dst = ta.DeviceBuffer(len(t["copies"]) * REGION)
+    dst = ta.DeviceBuffer(len(t["copies"]) * rg)
     dst.fill(0xAB)
     for k, ((_, _, _, _, off, inc), sb, e, o) in enumerate(
             zip(t["copies"], t["sbs"], t["exp"], t["offs"])):
         assert ta.pack_sb(sb, t["src"].addr(off), inc, pk.addr(o), e.size, 0, variant) == e.size
-        assert ta.unpack_sb(sb, t["psrc"].addr(o), e.size, dst.addr(k * REGION + off), inc, 0,
+        assert ta.unpack_sb(sb, t["psrc"].addr(o), e.size, dst.addr(k * rg + off), inc, 0,
                             variant) == e.size
     ta.synchronize()
     check(t, pk, dst, variant)
@@ -74,15 +76,15 @@ def test_every_tiny_copy_single_copy_kernel(tiny, variant):
 
 @pytest.mark.parametrize("variant", range(19))
 def test_every_tiny_copy_batch_kernel(tiny, variant):
-    t = tiny
+    t, rg = tiny, tiny["region"]
     pk = ta.DeviceBuffer(t["packed"].size)
     pk.fill(0x5A)
-    dst = ta.DeviceBuffer(len(t["copies"]) * REGION)
+    dst = ta.DeviceBuffer(len(t["copies"]) * rg)
     dst.fill(0xAB)
     packs, unpacks = [], []
     for k, ((_, _, _, _, off, inc), sb, o) in enumerate(zip(t["copies"], t["sbs"], t["offs"])):
         packs.append((sb, t["src"].addr(off), pk.addr(o), inc, ta.DIR_PACK, variant))
-        unpacks.append((sb, dst.addr(k * REGION + off), t["psrc"].addr(o), inc,
+        unpacks.append((sb, dst.addr(k * rg + off), t["psrc"].addr(o), inc,
                         ta.DIR_UNPACK, variant))
     bp, bu = ta.Batch(packs), ta.Batch(unpacks)
     bp.launch()
@@ -106,6 +108,6 @@ def test_every_tiny_copy_batch_kernel(tiny, variant):
     for j in sample:
         o, e = t["offs"][j], t["exp"][j]
         want_p[o:o + e.size] = e
-        want_u[j * REGION:(j + 1) * REGION] = t["image"][j * REGION:(j + 1) * REGION]
+        want_u[j * rg:(j + 1) * rg] = t["image"][j * rg:(j + 1) * rg]
     assert (pk.download() == want_p).all(), (variant, "small pack batches")
     assert (dst.download() == want_u).all(), (variant, "small unpack batches")

@@ -288,3 +288,27 @@ def tiny_copies():
                     out.append((0, extent, [bl, rows], [1, stride], off, 1))
                 out.append((0, extent + 4, [bl, rows], [1, stride], 5, 2))
     return out
+
+
+# Every tiny 3D byte copy of a coarser grid: rows of 1, 3, 4, 8, 12 or 16 B;
+# 1-3 rows at strides of L, L + 1 and L + 5 bytes; 1-3 planes at strides of
+# rows x stride, that plus 3, and minus (that plus 2) -- a negative plane
+# stride starts the type map (planes - 1) x |stride| bytes in; the strided
+# side at each offset 0-15.  Reach: at most 15 + 132 + 42 + 16 = 205 bytes.
+TINY3_SPAN = 256
+
+
+def tiny_copies_3d():
+    """[(start, extent, counts, strides, offset, incount)]."""
+    out = []
+    for bl in (1, 3, 4, 8, 12, 16):
+        for rows in range(1, 4):
+            for s1 in (bl, bl + 1, bl + 5):
+                for planes in range(1, 4):
+                    for s2 in (rows * s1, rows * s1 + 3, -(rows * s1 + 2)):
+                        start = (planes - 1) * -s2 if s2 < 0 else 0
+                        extent = start + (planes - 1) * max(s2, 0) + rows * s1
+                        for off in range(16):
+                            out.append((start, extent, [bl, rows, planes], [1, s1, s2],
+                                        off, 1))
+    return out
