@@ -407,6 +407,8 @@ typedef struct tempi_counters {
   int64_t host_staged_calls;    /* MPI_Pack / MPI_Unpack with one side in memory
                                    the other side's engine cannot address,
                                    staged through device or pinned host memory */
+  int64_t signalled_waits;      /* synchronous copies whose own completion
+                                   signal the host waited for */
 } tempi_counters;
 int tempi_counters_get(tempi_counters *out);
 int tempi_counters_reset(void);

@@ -114,6 +114,7 @@ class Counters(ctypes.Structure):
             "trial_launches",
             "method_model_choices",
             "host_staged_calls",
+            "signalled_waits",
         )
     ]
 

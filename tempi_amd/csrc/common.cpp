@@ -215,6 +215,7 @@ TEMPI_EXPORT int tempi_counters_get(tempi_counters *out) {
   out->trial_launches = c.trial_launches;
   out->method_model_choices = c.method_model_choices;
   out->host_staged_calls = c.host_staged_calls;
+  out->signalled_waits = c.signalled_waits;
   return TEMPI_SUCCESS;
 }
 
@@ -238,6 +239,7 @@ TEMPI_EXPORT int tempi_counters_reset(void) {
   c.trial_launches = 0;
   c.method_model_choices = 0;
   c.host_staged_calls = 0;
+  c.signalled_waits = 0;
   return TEMPI_SUCCESS;
 }
 

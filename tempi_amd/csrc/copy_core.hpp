@@ -42,6 +42,7 @@ struct KParams {
   uint32_t cnt[kMaxRowDims];
   FastDiv div[kMaxRowDims];
   int64_t str[kMaxRowDims];
+  uint32_t signal; // nonzero: the launch signals its completion (kernels.hip)
 };
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));

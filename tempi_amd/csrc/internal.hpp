@@ -112,6 +112,7 @@ struct Counters {
   std::atomic<int64_t> trial_launches{0}; // copies timed by the library
   std::atomic<int64_t> method_model_choices{0}; // staging methods the MLP chose
   std::atomic<int64_t> host_staged_calls{0}; // pack/unpack staged for pageable memory
+  std::atomic<int64_t> signalled_waits{0}; // copies waited for by their own signal
 };
 Counters &counters();
 // log every counter, in rank order, at LOG_DEBUG (runtime.cpp)
@@ -189,6 +190,7 @@ void normalize_dims(std::vector<Dim> &dims);
 bool have_device();           // lazily probes hipGetDeviceCount
 int ensure_runtime();         // stream etc.; TEMPI_SUCCESS or error
 hipStream_t library_stream(); // valid after ensure_runtime()
+int library_device();          // the device of library_stream()
 int device_cu_count();
 // returns 1 and the device-usable address when the kernels can read/write p
 int device_address(const void *p, void **dev);
@@ -231,6 +233,10 @@ bool geom_aligned(const Geom &g, const char *strided, const char *packed,
 // buffer origin (strided) and of the first packed byte.
 int launch_copy(const Geom &g, const char *strided, char *packed, int dir,
                 int variant, hipStream_t stream, int max_blocks = 0);
+// launch_copy on the library stream, then wait until the copy is done (a
+// copy of a few workgroups signals its own completion to the host)
+int launch_copy_wait(const Geom &g, const char *strided, char *packed, int dir,
+                     int variant, int max_blocks = 0);
 
 // closed-form variant rule (the "plain heuristic")
 int heuristic_variant(const Geom &g, const char *strided, const char *packed,

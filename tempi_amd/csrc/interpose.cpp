@@ -199,6 +199,22 @@ int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
   return launch_copy(g, strided, packed, dir, c.variant, stream, c.max_blocks);
 }
 
+// run_copy on the library stream, returning when the copy is done
+static int run_copy_wait(const tempi_strided_block &sb, int64_t count,
+                         const char *strided, char *packed, int dir) {
+  Geom g;
+  int err = make_geom(sb, count, &g);
+  if (err) {
+    return err;
+  }
+  if (g.rows == 0) {
+    return TEMPI_SUCCESS;
+  }
+  const Choice c = choose(g, strided, packed, dir, env().select,
+                          library_stream(), /*in_place=*/true);
+  return launch_copy_wait(g, strided, packed, dir, c.variant, c.max_blocks);
+}
+
 // A pinned, device-mapped host staging buffer from the pool, returned on
 // scope exit.
 struct HostStage {
@@ -216,7 +232,6 @@ struct HostStage {
 // memory would go through the runtime's own bounce buffer.
 int pack_or_unpack(const Canon &cn, int count, const char *strided,
                    char *packed, bool host_packed, int dir) {
-  hipStream_t s = library_stream();
   const int64_t bytes = (int64_t)count * cn.size;
   HostStage stage;
   char *kernel_packed = packed;
@@ -233,12 +248,9 @@ int pack_or_unpack(const Canon &cn, int count, const char *strided,
       memcpy(stage.p, packed, (size_t)bytes);
     }
   }
-  int err = run_copy(cn.sb, count, strided, kernel_packed, dir, -1, s);
-  // the staging buffer is read (or goes back to the pool) only after the
-  // stream is done
-  if (hipSuccess != hipStreamSynchronize(s) && !err) {
-    err = set_error(TEMPI_ERR_HIP, "stream synchronize failed");
-  }
+  // returns once the copy is done (also on failure: the staging buffer is
+  // read, or goes back to the pool, only after that)
+  const int err = run_copy_wait(cn.sb, count, strided, kernel_packed, dir);
   if (!err && host_packed && dir == TEMPI_DIR_PACK) {
     memcpy(packed, stage.p, (size_t)bytes);
   }

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 
 namespace tempi {
 
@@ -41,6 +42,48 @@ FastDiv make_fastdiv(uint32_t d) {
   return f;
 }
 
+// Where signalled copies count their finished workgroups and report to the
+// host; set once by the host (signal_ready) before the first signalled
+// launch.
+struct SignalSlots {
+  unsigned long long *counter;   // finished workgroups of the running launch
+  unsigned long long *completed; // signalled launches completed so far
+  unsigned long long *flag;      // `completed`, in pinned host memory
+};
+static __device__ SignalSlots g_signal_slots;
+
+// The end of a launch with p.signal set, run by every thread of the
+// workgroup: the workgroup's stores are released at system scope (the
+// barrier's workgroup-scope fences order the other waves' stores before
+// thread 0's release, scopes composing), then counted.  The launch's last
+// workgroup resets the counter for the next signalled launch -- launches on
+// the library stream never overlap -- and stores the number of completed
+// signalled launches to the host's flag.
+__device__ __forceinline__ void signal_done(const KParams &p) {
+  if (!p.signal) {
+    return;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    const SignalSlots s = g_signal_slots;
+    const unsigned long long done =
+        __hip_atomic_fetch_add(s.counter, 1ull, __ATOMIC_ACQ_REL,
+                               __HIP_MEMORY_SCOPE_AGENT) +
+        1ull;
+    if (done == gridDim.x) {
+      __hip_atomic_store(s.counter, 0ull, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long c =
+          __hip_atomic_fetch_add(s.completed, 1ull, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT) +
+          1ull;
+      __hip_atomic_store(s.flag, c, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+}
+
 template <int P, int W, int U, bool ALIGNED, int DIR, bool NTS, bool NTL>
 __global__ void __launch_bounds__(kBlock) k_strided_copy(KParams p) {
   constexpr uint32_t GROUPS = kBlock / P;
@@ -51,6 +94,7 @@ __global__ void __launch_bounds__(kBlock) k_strided_copy(KParams p) {
        u += step * K) {
     copy_units<P, W, U, ALIGNED, DIR, NTS, NTL>(p, u, step, p.units, lane);
   }
+  signal_done(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -328,9 +372,11 @@ static int for_each_launch(const Geom &g, const char *strided, char *packed,
   return TEMPI_SUCCESS;
 }
 
+// One launch; `signal`: it signals its completion (the caller holds the
+// signal's lock).
 static int launch_one(const Geom &g, const char *strided, char *packed,
                       int dir, int variant, hipStream_t stream,
-                      int max_blocks) {
+                      int max_blocks, bool signal = false) {
   KParams p;
   bool aligned = false;
   int err = fill_kparams(g, strided, packed, dir, variant, &p, &aligned);
@@ -339,6 +385,7 @@ static int launch_one(const Geom &g, const char *strided, char *packed,
   }
   const int64_t blocks = grid_of(variant, p.units, max_blocks, device_cu_count());
   KernelFn fn = kKernels[variant][aligned ? 1 : 0][dir];
+  p.signal = signal ? 1u : 0u;
   void *args[] = {&p};
   TEMPI_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void *>(fn),
                                   dim3((unsigned)blocks), dim3(kBlock), args,
@@ -374,6 +421,158 @@ int launch_copy(const Geom &g, const char *strided, char *packed, int dir,
                            return launch_one(pg, ps, pp, dir, variant, stream,
                                              max_blocks);
                          });
+}
+
+// ---------------------------------------------------------------------------
+// waiting for a synchronous copy
+// ---------------------------------------------------------------------------
+//
+// MPI_Pack and MPI_Unpack return when their copy is done.  On MI355X an
+// empty kernel's own store reaches the host 5.7 us after its launch begins,
+// but hipStreamSynchronize returns after 11.1 us: the rest is the runtime's
+// end-of-kernel processing.  A copy of one launch of at most
+// kSignalMaxBlocks workgroups on the library stream therefore signals its
+// own completion (CopySignal) and the host spins on the flag.  Larger grids
+// wait on the stream: each workgroup's system-scope release costs more than
+// the wait saves once there are a few hundred of them.
+
+namespace {
+
+constexpr int64_t kSignalMaxBlocks = 64;
+
+struct SignalState {
+  std::mutex mu; // held from a launch's sequence number to its launch
+  bool tried = false;
+  unsigned long long *slots = nullptr;    // device memory: counter, completed
+  unsigned long long *flag = nullptr;     // pinned host memory
+  unsigned long long issued = 0;          // signalled launches so far
+  // set after a signalled wait fails: the counter may no longer match the
+  // launches, so later copies wait on the stream
+  std::atomic<bool> off{false};
+};
+
+SignalState &signal_state() {
+  static SignalState s;
+  return s;
+}
+
+// Counter, completed count and flag, allocated and published to the kernels
+// once, on the library stream's device (the state's lock held); false when
+// they cannot be.
+bool signal_ready(SignalState &st) {
+  if (st.tried) {
+    return st.flag != nullptr && !st.off.load(std::memory_order_relaxed);
+  }
+  st.tried = true;
+  void *c = nullptr, *f = nullptr, *fd = nullptr;
+  bool ok = hipSuccess == hipMalloc(&c, 256) && hipSuccess == hipMemset(c, 0, 256) &&
+            hipSuccess == hipHostMalloc(&f, 256, hipHostMallocMapped |
+                                                     hipHostMallocPortable) &&
+            hipSuccess == hipHostGetDevicePointer(&fd, f, 0);
+  if (ok) {
+    __atomic_store_n(static_cast<unsigned long long *>(f), 0ull, __ATOMIC_RELEASE);
+    unsigned long long *cs = static_cast<unsigned long long *>(c);
+    const SignalSlots slots{cs, cs + 1, static_cast<unsigned long long *>(fd)};
+    ok = hipSuccess == hipMemcpyToSymbol(HIP_SYMBOL(g_signal_slots), &slots,
+                                         sizeof slots);
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    if (c) {
+      (void)hipFree(c);
+    }
+    if (f) {
+      (void)hipHostFree(f);
+    }
+    TEMPI_LOG(LOG_WARN, "no completion signal: synchronous copies wait on the stream");
+    return false;
+  }
+  st.slots = static_cast<unsigned long long *>(c);
+  st.flag = static_cast<unsigned long long *>(f);
+  return true;
+}
+
+// Spin until the flag counts `seq` completed signalled launches.  A launch
+// that never runs to its end leaves the flag short, so after 0.2 ms the
+// stream is checked every 0.1 ms: finished without the flag is an error,
+// and so is a failed stream.
+int wait_signal(const SignalState &st, unsigned long long seq,
+                hipStream_t stream) {
+  const auto t0 = std::chrono::steady_clock::now();
+  double check_at = 2e-4;
+  for (;;) {
+    if (__atomic_load_n(st.flag, __ATOMIC_ACQUIRE) >= seq) {
+      return TEMPI_SUCCESS;
+    }
+    __builtin_ia32_pause();
+    const double el = std::chrono::duration<double>(
+                          std::chrono::steady_clock::now() - t0)
+                          .count();
+    if (el < check_at) {
+      continue;
+    }
+    const hipError_t e = hipStreamQuery(stream);
+    if (e == hipSuccess) {
+      if (__atomic_load_n(st.flag, __ATOMIC_ACQUIRE) >= seq) {
+        return TEMPI_SUCCESS;
+      }
+      return set_error(TEMPI_ERR_HIP, "a copy ended without its signal");
+    }
+    if (e != hipErrorNotReady) {
+      (void)hipGetLastError();
+      return set_error(TEMPI_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
+    }
+    check_at = el + 1e-4;
+  }
+}
+
+} // namespace
+
+int launch_copy_wait(const Geom &g, const char *strided, char *packed,
+                     int dir, int variant, int max_blocks) {
+  if (g.rows == 0 || g.L == 0) {
+    return TEMPI_SUCCESS;
+  }
+  if (variant < 0 || variant >= kNumVariants) {
+    return set_error(TEMPI_ERR_ARG, "variant %d out of range", variant);
+  }
+  hipStream_t s = library_stream();
+  const int64_t units = g.rows * segments_of(g, variant);
+  int dev = -1;
+  const bool small =
+      g.L <= (1ll << 30) && units < kMaxUnits &&
+      grid_of(variant, (uint32_t)units, max_blocks, device_cu_count()) <=
+          kSignalMaxBlocks &&
+      hipSuccess == hipGetDevice(&dev) && dev == library_device();
+  if (small) {
+    SignalState &st = signal_state();
+    std::unique_lock<std::mutex> lk(st.mu);
+    if (signal_ready(st)) {
+      int err = launch_one(g, strided, packed, dir, variant, s, max_blocks,
+                           /*signal=*/true);
+      const unsigned long long seq = err ? st.issued : ++st.issued;
+      lk.unlock();
+      if (!err) {
+        err = wait_signal(st, seq, s);
+        if (err) {
+          st.off.store(true);
+          TEMPI_LOG(LOG_WARN, "completion signal switched off: %s",
+                    tempi_last_error());
+        }
+      }
+      if (err) {
+        (void)hipStreamSynchronize(s);
+        return err;
+      }
+      ++counters().signalled_waits;
+      return TEMPI_SUCCESS;
+    }
+  }
+  int err = launch_copy(g, strided, packed, dir, variant, s, max_blocks);
+  if (hipSuccess != hipStreamSynchronize(s) && !err) {
+    err = set_error(TEMPI_ERR_HIP, "stream synchronize failed");
+  }
+  return err;
 }
 
 // ---------------------------------------------------------------------------

@@ -54,6 +54,7 @@ void report_counters() {
       {"trial_launches", c.trial_launches},
       {"method_model_choices", c.method_model_choices},
       {"host_staged_calls", c.host_staged_calls},
+      {"signalled_waits", c.signalled_waits},
   };
   for (int r = 0; r < size; ++r) {
     if (r == rank) {
@@ -75,6 +76,7 @@ struct Runtime {
   std::mutex mu;
   std::atomic<bool> ready{false};
   hipStream_t stream = nullptr;
+  int device = 0;
   int cus = 0;
 };
 Runtime &rt() {
@@ -113,6 +115,7 @@ int ensure_runtime() {
   hipDeviceProp_t prop;
   TEMPI_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
   r.cus = prop.multiProcessorCount;
+  r.device = dev;
   // default priority, non-blocking: no implicit ordering with the caller's
   // streams, as the reference's kernStream
   TEMPI_HIP_CHECK(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
@@ -122,6 +125,8 @@ int ensure_runtime() {
 }
 
 hipStream_t library_stream() { return rt().stream; }
+
+int library_device() { return rt().device; }
 
 int device_cu_count() { return rt().cus > 0 ? rt().cus : 256; }
 

