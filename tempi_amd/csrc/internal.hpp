@@ -400,6 +400,10 @@ struct TraceRange {
 // variant choice (variant < 0) or the given one; device addresses
 int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
              char *packed, int dir, int variant, hipStream_t stream);
+// run_copy on the library stream with the library's variant, returning when
+// the copy is done (launch_copy_wait)
+int run_copy_wait(const tempi_strided_block &sb, int64_t count,
+                  const char *strided, char *packed, int dir);
 
 // cached staging buffers (pool.cpp): device memory, or pinned host memory
 // mapped into the device address space

@@ -200,7 +200,7 @@ int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
 }
 
 // run_copy on the library stream, returning when the copy is done
-static int run_copy_wait(const tempi_strided_block &sb, int64_t count,
+int run_copy_wait(const tempi_strided_block &sb, int64_t count,
                          const char *strided, char *packed, int dir) {
   Geom g;
   int err = make_geom(sb, count, &g);

@@ -808,11 +808,15 @@ TEMPI_EXPORT int MPI_Send(const void *buf, int count, MPI_Datatype datatype,
   Staging st;
   hipStream_t s = library_stream();
   int terr = st.acquire(method, bytes);
-  if (!terr) {
+  if (!terr && method != TEMPI_METHOD_STAGED) {
+    // the pack is all there is to wait for
+    terr = run_copy_wait(cn.sb, count, static_cast<const char *>(dev),
+                         st.kernel_side(), TEMPI_DIR_PACK);
+  } else if (!terr) {
     terr = enqueue_pack(cn, count, dev, st, s);
-  }
-  if (!terr && hipSuccess != hipStreamSynchronize(s)) {
-    terr = set_error(TEMPI_ERR_HIP, "pack for MPI_Send failed");
+    if (!terr && hipSuccess != hipStreamSynchronize(s)) {
+      terr = set_error(TEMPI_ERR_HIP, "pack for MPI_Send failed");
+    }
   }
   if (terr) {
     st.release();
@@ -878,9 +882,15 @@ TEMPI_EXPORT int MPI_Recv(void *buf, int count, MPI_Datatype datatype,
   if (err == MPI_SUCCESS) {
     const int n = received_objects(local, cn.size, count);
     hipStream_t s = library_stream();
-    int terr = enqueue_unpack(cn, n, dev, st, (int64_t)n * cn.size, s);
-    if (!terr && hipSuccess != hipStreamSynchronize(s)) {
-      terr = set_error(TEMPI_ERR_HIP, "unpack for MPI_Recv failed");
+    int terr = TEMPI_SUCCESS;
+    if (method != TEMPI_METHOD_STAGED) {
+      terr = run_copy_wait(cn.sb, n, static_cast<const char *>(dev),
+                           st.kernel_side(), TEMPI_DIR_UNPACK);
+    } else {
+      terr = enqueue_unpack(cn, n, dev, st, (int64_t)n * cn.size, s);
+      if (!terr && hipSuccess != hipStreamSynchronize(s)) {
+        terr = set_error(TEMPI_ERR_HIP, "unpack for MPI_Recv failed");
+      }
     }
     if (terr) {
       TEMPI_LOG(LOG_ERROR, "MPI_Recv: %s", tempi_last_error());

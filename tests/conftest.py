@@ -48,3 +48,20 @@ def spec_from_json(s):
         return ("dup", spec_from_json(s[1]))
     # contiguous / vector / hvector / resized: scalars then the child
     return tuple(s[:-1]) + (spec_from_json(s[-1]),)
+
+
+def signalled(sb, strided, packed, direction):
+    """Whether the library's copy of one object runs as one launch of at most
+    64 workgroups -- the copies that signal their own completion -- from the
+    variant it picks and the grid its launch code forms (256-thread
+    workgroups of 256 / lanes-per-row groups, each moving 1 unit, or 4 / U
+    units when U < 4, at most 16 workgroups per CU of MI355X's 256)."""
+    import tempi_amd as ta
+
+    v = ta.select_variant(sb, 1, strided, packed, direction, ta.SELECT_DEFAULT)
+    info = ta.variant_info(v)
+    e = ta.copy_emulate(sb, 1, strided, packed, direction, v)
+    k = 1 if info["unroll"] >= 4 else 4 // info["unroll"]
+    groups = (256 // info["lanes_per_row"]) * k
+    blocks = min(-(-e["max_units"] // groups), 16 * 256)
+    return e["launches"] == 1 and blocks <= 64

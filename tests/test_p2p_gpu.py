@@ -8,6 +8,7 @@ import pytest
 
 import tempi_amd as ta
 import typecatalog as tc
+from conftest import signalled
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -98,9 +99,13 @@ def test_wire_format_is_the_libraries_packed_form(mpi, method):
 
 @pytest.mark.parametrize("method", [ONESHOT, STAGED])
 def test_blocking_send_and_recv_self(mpi, method):
+    """Blocking sends and receives; with ONESHOT a pack or unpack of a few
+    workgroups returns on its own completion signal (STAGED copies to the
+    host after the kernel and waits on the stream)."""
     ta.lib().tempi_set_datatype_method(method)
     spec = ("vector", 4096, 48, 200, tc.BYTE)
     h, src, dst, expected = setup(mpi, spec, 1, 5)
+    sig0 = ta.counters()["signalled_waits"]
     # blocking send matched by a receive posted first
     err, rr = mpi.irecv(dst.addr(), 1, h, 0, 21)
     assert err == 0
@@ -117,6 +122,15 @@ def test_blocking_send_and_recv_self(mpi, method):
     err, _, sr = mpi.wait(sr)
     assert err == 0
     assert (dst.download() == expected).all()
+    # the blocking send's pack and the blocking receive's unpack (the
+    # staging buffer's device address decides only alignment, 256-B here)
+    sb, _ = mpi.strided_block(h)
+    stage = ta.DeviceBuffer(sb.size())
+    want = 0
+    if method == ONESHOT:
+        want = (signalled(sb, src.addr(), stage.addr(), ta.DIR_PACK)
+                + signalled(sb, dst.addr(), stage.addr(), ta.DIR_UNPACK))
+    assert ta.counters()["signalled_waits"] - sig0 == want
     mpi.free(h)
 
 

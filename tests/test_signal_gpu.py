@@ -11,6 +11,7 @@ import pytest
 
 import tempi_amd as ta
 import typecatalog as tc
+from conftest import signalled
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -24,21 +25,6 @@ def hip():
     lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
     lib.hipMemcpy.restype = ctypes.c_int
     return lib
-
-
-def signalled(sb, strided, packed, direction):
-    """Whether the library's copy of one object runs as one launch of at most
-    64 workgroups -- the copies that signal their own completion -- from the
-    variant it picks and the grid its launch code forms (256-thread
-    workgroups of 256 / lanes-per-row groups, each moving 1 unit, or 4 / U
-    units when U < 4, at most 16 workgroups per CU of MI355X's 256)."""
-    v = ta.select_variant(sb, 1, strided, packed, direction, ta.SELECT_DEFAULT)
-    info = ta.variant_info(v)
-    e = ta.copy_emulate(sb, 1, strided, packed, direction, v)
-    k = 1 if info["unroll"] >= 4 else 4 // info["unroll"]
-    groups = (256 // info["lanes_per_row"]) * k
-    blocks = min(-(-e["max_units"] // groups), 16 * 256)
-    return e["launches"] == 1 and blocks <= 64
 
 
 def fetch(hip, buf, n):
