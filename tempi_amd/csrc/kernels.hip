@@ -432,13 +432,15 @@ int launch_copy(const Geom &g, const char *strided, char *packed, int dir,
 // but hipStreamSynchronize returns after 11.1 us: the rest is the runtime's
 // end-of-kernel processing.  A copy of one launch of at most
 // kSignalMaxBlocks workgroups on the library stream therefore signals its
-// own completion (CopySignal) and the host spins on the flag.  Larger grids
-// wait on the stream: each workgroup's system-scope release costs more than
-// the wait saves once there are a few hundred of them.
+// own completion (KParams::signal) and the host spins on the flag.  Larger
+// grids wait on the stream: the workgroups' releases and counter updates
+// queue behind each other, and past about 150 of them they cost more than
+// the wait saves (MPI_Pack of 64-B rows: 12.5-13.1 us signalled against
+// 13.3-13.7 us at 128 workgroups, 14.9 against 13.7 us at 192).
 
 namespace {
 
-constexpr int64_t kSignalMaxBlocks = 64;
+constexpr int64_t kSignalMaxBlocks = 128;
 
 struct SignalState {
   std::mutex mu; // held from a launch's sequence number to its launch

@@ -52,7 +52,7 @@ def spec_from_json(s):
 
 def signalled(sb, strided, packed, direction):
     """Whether the library's copy of one object runs as one launch of at most
-    64 workgroups -- the copies that signal their own completion -- from the
+    128 workgroups -- the copies that signal their own completion -- from the
     variant it picks and the grid its launch code forms (256-thread
     workgroups of 256 / lanes-per-row groups, each moving 1 unit, or 4 / U
     units when U < 4, at most 16 workgroups per CU of MI355X's 256)."""
@@ -64,4 +64,4 @@ def signalled(sb, strided, packed, direction):
     k = 1 if info["unroll"] >= 4 else 4 // info["unroll"]
     groups = (256 // info["lanes_per_row"]) * k
     blocks = min(-(-e["max_units"] // groups), 16 * 256)
-    return e["launches"] == 1 and blocks <= 64
+    return e["launches"] == 1 and blocks <= 128

@@ -90,7 +90,7 @@ def test_packs_into_pageable_memory(mpi):
 
 
 def test_large_copies_wait_on_the_stream(mpi, hip):
-    """A copy of more than 64 workgroups waits with the stream (its result is
+    """A copy of more than 128 workgroups waits with the stream (its result is
     checked the same way)."""
     spec = ("vector", 1 << 16, 64, 96, tc.BYTE)  # 4 MiB in 64-B rows
     h = mpi.type_from_spec(spec)
