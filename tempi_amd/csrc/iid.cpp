@@ -248,16 +248,23 @@ int iid_test(const double *samples, int64_t n, uint64_t seed,
       }
     }
   };
-  if (workers == 1) {
-    work(0);
-  } else {
-    std::vector<std::thread> th;
-    for (int w = 0; w < workers; ++w) {
+  // worker w takes shuffles w, w + workers, ...; a worker whose thread
+  // cannot be started runs on this one (the counts are the same either way)
+  std::vector<std::thread> th;
+  int started = 1;
+  try {
+    for (int w = 1; w < workers; ++w) {
       th.emplace_back(work, w);
+      ++started;
     }
-    for (std::thread &x : th) {
-      x.join();
-    }
+  } catch (...) {
+  }
+  work(0);
+  for (int w = started; w < workers; ++w) {
+    work(w);
+  }
+  for (std::thread &x : th) {
+    x.join();
   }
   bool iid = true;
   for (int s = 0; s < kIidStats; ++s) {

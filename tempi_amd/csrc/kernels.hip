@@ -466,8 +466,13 @@ bool signal_ready(SignalState &st) {
     return st.flag != nullptr && !st.off.load(std::memory_order_relaxed);
   }
   st.tried = true;
+  // the zeroed counters and the published slots are in place before the
+  // first signalled launch: both go through the library stream (which does
+  // not wait for the null stream), and it is drained here
+  hipStream_t s = library_stream();
   void *c = nullptr, *f = nullptr, *fd = nullptr;
-  bool ok = hipSuccess == hipMalloc(&c, 256) && hipSuccess == hipMemset(c, 0, 256) &&
+  bool ok = hipSuccess == hipMalloc(&c, 256) &&
+            hipSuccess == hipMemsetAsync(c, 0, 256, s) &&
             hipSuccess == hipHostMalloc(&f, 256, hipHostMallocMapped |
                                                      hipHostMallocPortable) &&
             hipSuccess == hipHostGetDevicePointer(&fd, f, 0);
@@ -475,8 +480,10 @@ bool signal_ready(SignalState &st) {
     __atomic_store_n(static_cast<unsigned long long *>(f), 0ull, __ATOMIC_RELEASE);
     unsigned long long *cs = static_cast<unsigned long long *>(c);
     const SignalSlots slots{cs, cs + 1, static_cast<unsigned long long *>(fd)};
-    ok = hipSuccess == hipMemcpyToSymbol(HIP_SYMBOL(g_signal_slots), &slots,
-                                         sizeof slots);
+    ok = hipSuccess == hipMemcpyToSymbolAsync(HIP_SYMBOL(g_signal_slots), &slots,
+                                              sizeof slots, 0,
+                                              hipMemcpyHostToDevice, s) &&
+         hipSuccess == hipStreamSynchronize(s);
   }
   if (!ok) {
     (void)hipGetLastError();
