@@ -237,6 +237,15 @@ int launch_copy(const Geom &g, const char *strided, char *packed, int dir,
 // copy of a few workgroups signals its own completion to the host)
 int launch_copy_wait(const Geom &g, const char *strided, char *packed, int dir,
                      int variant, int max_blocks = 0);
+// launch_copy on the library stream without waiting; a copy that signals
+// its completion gets *seq > 0, to test with signal_reached (else 0)
+int launch_copy_signalled(const Geom &g, const char *strided, char *packed,
+                          int dir, int variant, int max_blocks,
+                          unsigned long long *seq);
+bool signal_reached(unsigned long long seq);
+// until signal `seq` arrives (seq 0: none) or `ev`, recorded after the
+// copy, completes; a failed event is an error
+int wait_signal_or_event(unsigned long long seq, hipEvent_t ev);
 
 // closed-form variant rule (the "plain heuristic")
 int heuristic_variant(const Geom &g, const char *strided, const char *packed,
@@ -404,6 +413,10 @@ int run_copy(const tempi_strided_block &sb, int64_t count, const char *strided,
 // the copy is done (launch_copy_wait)
 int run_copy_wait(const tempi_strided_block &sb, int64_t count,
                   const char *strided, char *packed, int dir);
+// the same without waiting; *seq > 0 when the copy signals its completion
+int run_copy_signalled(const tempi_strided_block &sb, int64_t count,
+                       const char *strided, char *packed, int dir,
+                       unsigned long long *seq);
 
 // cached staging buffers (pool.cpp): device memory, or pinned host memory
 // mapped into the device address space

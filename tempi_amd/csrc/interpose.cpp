@@ -215,6 +215,26 @@ int run_copy_wait(const tempi_strided_block &sb, int64_t count,
   return launch_copy_wait(g, strided, packed, dir, c.variant, c.max_blocks);
 }
 
+// run_copy on the library stream without waiting; *seq > 0 when the copy
+// signals its completion (signal_reached)
+int run_copy_signalled(const tempi_strided_block &sb, int64_t count,
+                       const char *strided, char *packed, int dir,
+                       unsigned long long *seq) {
+  *seq = 0;
+  Geom g;
+  int err = make_geom(sb, count, &g);
+  if (err) {
+    return err;
+  }
+  if (g.rows == 0) {
+    return TEMPI_SUCCESS;
+  }
+  const Choice c = choose(g, strided, packed, dir, env().select,
+                          library_stream(), /*in_place=*/true);
+  return launch_copy_signalled(g, strided, packed, dir, c.variant,
+                               c.max_blocks, seq);
+}
+
 // A pinned, device-mapped host staging buffer from the pool, returned on
 // scope exit.
 struct HostStage {

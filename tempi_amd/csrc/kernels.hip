@@ -535,7 +535,76 @@ int wait_signal(const SignalState &st, unsigned long long seq,
   }
 }
 
+// one launch of at most kSignalMaxBlocks workgroups, on the library
+// stream's device
+bool signal_fits(const Geom &g, int variant, int max_blocks) {
+  const int64_t units = g.rows * segments_of(g, variant);
+  int dev = -1;
+  return g.L <= (1ll << 30) && units < kMaxUnits &&
+         grid_of(variant, (uint32_t)units, max_blocks, device_cu_count()) <=
+             kSignalMaxBlocks &&
+         hipSuccess == hipGetDevice(&dev) && dev == library_device();
+}
+
 } // namespace
+
+int launch_copy_signalled(const Geom &g, const char *strided, char *packed,
+                          int dir, int variant, int max_blocks,
+                          unsigned long long *seq) {
+  *seq = 0;
+  if (g.rows == 0 || g.L == 0) {
+    return TEMPI_SUCCESS;
+  }
+  if (variant < 0 || variant >= kNumVariants) {
+    return set_error(TEMPI_ERR_ARG, "variant %d out of range", variant);
+  }
+  hipStream_t s = library_stream();
+  if (signal_fits(g, variant, max_blocks)) {
+    SignalState &st = signal_state();
+    std::lock_guard<std::mutex> lk(st.mu);
+    if (signal_ready(st)) {
+      const int err = launch_one(g, strided, packed, dir, variant, s,
+                                 max_blocks, /*signal=*/true);
+      if (!err) {
+        *seq = ++st.issued;
+      }
+      return err;
+    }
+  }
+  return launch_copy(g, strided, packed, dir, variant, s, max_blocks);
+}
+
+bool signal_reached(unsigned long long seq) {
+  const SignalState &st = signal_state();
+  return seq && st.flag && __atomic_load_n(st.flag, __ATOMIC_ACQUIRE) >= seq;
+}
+
+int wait_signal_or_event(unsigned long long seq, hipEvent_t ev) {
+  const auto t0 = std::chrono::steady_clock::now();
+  double check_at = seq ? 2e-4 : 0.0;
+  for (;;) {
+    if (signal_reached(seq)) {
+      ++counters().signalled_waits;
+      return TEMPI_SUCCESS;
+    }
+    __builtin_ia32_pause();
+    const double el = std::chrono::duration<double>(
+                          std::chrono::steady_clock::now() - t0)
+                          .count();
+    if (el < check_at) {
+      continue;
+    }
+    const hipError_t q = seq ? hipEventQuery(ev) : hipEventSynchronize(ev);
+    if (q == hipSuccess) {
+      return TEMPI_SUCCESS;
+    }
+    if (q != hipErrorNotReady) {
+      (void)hipGetLastError();
+      return set_error(TEMPI_ERR_HIP, "copy failed: %s", hipGetErrorString(q));
+    }
+    check_at = el + 1e-4;
+  }
+}
 
 int launch_copy_wait(const Geom &g, const char *strided, char *packed,
                      int dir, int variant, int max_blocks) {
@@ -546,14 +615,7 @@ int launch_copy_wait(const Geom &g, const char *strided, char *packed,
     return set_error(TEMPI_ERR_ARG, "variant %d out of range", variant);
   }
   hipStream_t s = library_stream();
-  const int64_t units = g.rows * segments_of(g, variant);
-  int dev = -1;
-  const bool small =
-      g.L <= (1ll << 30) && units < kMaxUnits &&
-      grid_of(variant, (uint32_t)units, max_blocks, device_cu_count()) <=
-          kSignalMaxBlocks &&
-      hipSuccess == hipGetDevice(&dev) && dev == library_device();
-  if (small) {
+  if (signal_fits(g, variant, max_blocks)) {
     SignalState &st = signal_state();
     std::unique_lock<std::mutex> lk(st.mu);
     if (signal_ready(st)) {

@@ -354,6 +354,7 @@ class IsendOp : public AsyncOp {
   enum { PACKING, SENDING, DONE } state_ = PACKING;
   Staging stg_;
   hipEvent_t ev_ = nullptr;
+  unsigned long long seq_ = 0; // the pack's completion signal (0: none)
   MPI_Request lib_req_ = MPI_REQUEST_NULL;
   int64_t bytes_;
   int peer_, tag_;
@@ -382,7 +383,11 @@ public:
   int start(const Canon &cn, int count, const void *ubuf_dev, int method) {
     hipStream_t s = library_stream();
     int err = stg_.acquire(method, bytes_);
-    if (!err) {
+    if (!err && method != TEMPI_METHOD_STAGED) {
+      // the pack is all there is to wait for: it may signal its completion
+      err = run_copy_signalled(cn.sb, count, static_cast<const char *>(ubuf_dev),
+                               stg_.kernel_side(), TEMPI_DIR_PACK, &seq_);
+    } else if (!err) {
       err = enqueue_pack(cn, count, ubuf_dev, stg_, s);
     }
     if (!err && hipSuccess != hipEventCreateWithFlags(&ev_, hipEventDisableTiming)) {
@@ -396,6 +401,10 @@ public:
   int wake() override {
     if (state_ != PACKING) {
       return MPI_SUCCESS;
+    }
+    if (signal_reached(seq_)) {
+      ++counters().signalled_waits;
+      return post();
     }
     const hipError_t q = hipEventQuery(ev_);
     if (q == hipErrorNotReady) {
@@ -412,8 +421,8 @@ public:
     if (state_ != PACKING) {
       return MPI_SUCCESS;
     }
-    if (hipSuccess != hipEventSynchronize(ev_)) {
-      set_error(TEMPI_ERR_HIP, "pack for MPI_Isend failed");
+    if (wait_signal_or_event(seq_, ev_)) {
+      TEMPI_LOG(LOG_ERROR, "pack for MPI_Isend: %s", tempi_last_error());
       return MPI_ERR_INTERN;
     }
     return post();
@@ -446,6 +455,7 @@ class IrecvOp : public AsyncOp {
   enum { RECEIVING, UNPACKING, DONE } state_ = RECEIVING;
   Staging stg_;
   hipEvent_t ev_ = nullptr;
+  unsigned long long seq_ = 0; // the unpack's completion signal (0: none)
   MPI_Request lib_req_ = MPI_REQUEST_NULL;
   MPI_Status status_{};
   Canon cn_;
@@ -461,7 +471,13 @@ class IrecvOp : public AsyncOp {
     }
     const int n = received_objects(status_, cn_.size, count_);
     hipStream_t s = library_stream();
-    int err = enqueue_unpack(cn_, n, ubuf_dev_, stg_, (int64_t)n * cn_.size, s);
+    int err = TEMPI_SUCCESS;
+    if (stg_.method != TEMPI_METHOD_STAGED) {
+      err = run_copy_signalled(cn_.sb, n, static_cast<const char *>(ubuf_dev_),
+                               stg_.kernel_side(), TEMPI_DIR_UNPACK, &seq_);
+    } else {
+      err = enqueue_unpack(cn_, n, ubuf_dev_, stg_, (int64_t)n * cn_.size, s);
+    }
     if (!err && hipSuccess != hipEventRecord(ev_, s)) {
       err = set_error(TEMPI_ERR_HIP, "event record failed");
     }
@@ -509,6 +525,11 @@ public:
         state_ = DONE;
         return error_;
       }
+      if (signal_reached(seq_)) {
+        ++counters().signalled_waits;
+        state_ = DONE;
+        return MPI_SUCCESS;
+      }
       const hipError_t q = hipEventQuery(ev_);
       if (q == hipSuccess) {
         state_ = DONE;
@@ -543,8 +564,8 @@ public:
       }
     }
     if (state_ == UNPACKING && error_ == MPI_SUCCESS &&
-        hipSuccess != hipEventSynchronize(ev_)) {
-      set_error(TEMPI_ERR_HIP, "unpack for MPI_Irecv failed");
+        wait_signal_or_event(seq_, ev_)) {
+      TEMPI_LOG(LOG_ERROR, "unpack for MPI_Irecv: %s", tempi_last_error());
       error_ = MPI_ERR_INTERN;
     }
     state_ = DONE;

@@ -61,6 +61,48 @@ def test_isend_irecv_self(mpi, method):
     mpi.free(h)
 
 
+def test_small_isend_irecv_complete_on_the_signal(mpi):
+    """ONESHOT Isend/Irecv of a small type: a pack or unpack of a few
+    workgroups completes its request as soon as its own completion signal
+    arrives.  The received bytes are read with a plain hipMemcpy on the null
+    stream right after MPI_Waitall returns, 50 rounds of fresh bytes."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    ta.lib().tempi_set_datatype_method(ONESHOT)
+    spec = ("vector", 300, 3, 7, tc.FLOAT)
+    h = mpi.type_from_spec(spec)
+    sb, size = mpi.strided_block(h)
+    o = orc.OracleType(spec)
+    lo, hi = o.span(1)
+    src, dst = ta.DeviceBuffer(hi), ta.DeviceBuffer(hi)
+    stage = ta.DeviceBuffer(size)  # stands in for the staging buffer's alignment
+    n_sig = (signalled(sb, src.addr(), stage.addr(), ta.DIR_PACK)
+             + signalled(sb, dst.addr(), stage.addr(), ta.DIR_UNPACK))
+    assert n_sig == 2
+    c0 = ta.counters()["signalled_waits"]
+    for it in range(50):
+        host = orc.splitmix_bytes(hi, 500 + it)
+        src.upload(host)
+        dst.fill(0xAB)
+        err, rr = mpi.irecv(dst.addr(), 1, h, 0, 31)
+        assert err == 0
+        err, sr = mpi.isend(src.addr(), 1, h, 0, 31)
+        assert err == 0
+        err, sts, reqs = mpi.waitall([rr, sr], [h, h])
+        assert err == 0 and sts[0]["count"] == 1
+        got = np.empty(hi, np.uint8)
+        assert hip.hipMemcpy(got.ctypes.data, dst.addr(), hi, 2) == 0
+        ref = np.full(hi, 0xAB, np.uint8)
+        o.unpack(o.pack(host, 0, 1), ref, 0, 1)
+        assert (got == ref).all(), it
+    # each request completes on its signal unless its event was seen first
+    done = ta.counters()["signalled_waits"] - c0
+    assert 0.9 * 50 * n_sig <= done <= 50 * n_sig, done
+    mpi.free(h)
+
+
 @pytest.mark.parametrize("method", [ONESHOT, STAGED])
 def test_wire_format_is_the_libraries_packed_form(mpi, method):
     """A message travels as MPI_PACKED bytes in type-map order, the MPI
