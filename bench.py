@@ -354,6 +354,7 @@ def main():
         extras["vector_sweep"] = bench_sweep(ta, m, tc, stream)
         extras["selector_regret"] = bench_selector_regret(ta)
         extras["isend_irecv_faces"] = bench_p2p(ta, m, faces, grid, grid2, host)
+        extras["p2p_small_latency"] = bench_p2p_small(ta, m, tc)
         extras["neighbor_alltoallw_halo"] = bench_neighbor(ta, m, tc)
 
         extras["reference_mpi_pack_sweep"] = bench_reference_sweep()
@@ -1319,6 +1320,68 @@ def bench_selector_regret(ta, n=512, rounds=3):
     src.free()
     dst.free()
     log(f"selector regret: { {k: v for k, v in res['ge_256KiB'].items() if k != 'copies'} }")
+    return res
+
+
+def bench_p2p_small(ta, m, tc, rounds=300):
+    """Round trips to self of small non-contiguous device messages with the
+    ONESHOT method (medians of `rounds`): MPI_Isend + MPI_Irecv +
+    MPI_Waitall, and a blocking MPI_Send into a posted MPI_Irecv.  Packs and
+    unpacks this small return on their own completion signal
+    (`signalled_waits`); the received bytes are checked once."""
+    import numpy as np
+
+    ta.lib().tempi_set_datatype_method(2)  # ONESHOT
+    res = {}
+    try:
+        for name, spec in (("vector_300x12B", ("vector", 300, 3, 7, tc.FLOAT)),
+                           ("vector_1024x256B", ("vector", 1024, 256, 512, tc.BYTE))):
+            h = m.type_from_spec(spec)
+            sb, size = m.strided_block(h)
+            span = sb.extent + 64
+            src, dst = ta.DeviceBuffer(span), ta.DeviceBuffer(span)
+            host = np.arange(span, dtype=np.uint32).astype(np.uint8)
+            src.upload(host)
+            dst.fill(0xAB)
+
+            def nonblocking():
+                _, rr = m.irecv(dst.addr(), 1, h, 0, 41)
+                _, sr = m.isend(src.addr(), 1, h, 0, 41)
+                m.waitall([rr, sr], [h, h])
+
+            def blocking():
+                _, rr = m.irecv(dst.addr(), 1, h, 0, 42)
+                m.send(src.addr(), 1, h, 0, 42)
+                m.wait(rr, h)
+
+            nonblocking()
+            got = dst.download()
+            sig0 = ta.counters()["signalled_waits"]
+            row = {}
+            for key, f in (("isend_irecv_waitall_us", nonblocking),
+                           ("send_into_posted_irecv_us", blocking)):
+                for _ in range(30):
+                    f()
+                ts = []
+                for _ in range(rounds):
+                    t0 = time.perf_counter()
+                    f()
+                    ts.append(time.perf_counter() - t0)
+                row[key] = round(float(np.median(ts)) * 1e6, 2)
+            row["signalled_completions_per_round"] = round(
+                (ta.counters()["signalled_waits"] - sig0) / (2 * (rounds + 30)), 2)
+            # the received bytes: the sent type map, 0xAB between its blocks
+            ref = np.full(span, 0xAB, np.uint8)
+            packed = np.zeros(size, np.uint8)
+            err, _ = m.pack(host.ctypes.data, 1, h, packed.ctypes.data, size, library=True)
+            err2, _ = m.unpack(packed.ctypes.data, size, ref.ctypes.data, 1, h, library=True)
+            ok = err == 0 and err2 == 0 and bool((got == ref).all())
+            row["bit_exact"] = ok
+            res[name] = row
+            m.free(h)
+    finally:
+        ta.lib().tempi_set_datatype_method(0)
+    log(f"p2p small latency: {res}")
     return res
 
 
