@@ -288,15 +288,25 @@ int iid_test(const double *samples, int64_t n, uint64_t seed,
 
 using namespace tempi;
 
+// (no exception leaves the C ABI: a failed allocation is TEMPI_ERR_NOMEM)
 TEMPI_EXPORT int tempi_iid_statistics(const double *samples, int64_t n,
                                       double *stats) {
   if (!stats) {
     return set_error(TEMPI_ERR_ARG, "null statistics array");
   }
-  return iid_statistics(samples, n, stats);
+  try {
+    return iid_statistics(samples, n, stats);
+  } catch (...) {
+    return set_error(TEMPI_ERR_NOMEM, "IID statistics of %lld samples",
+                     (long long)n);
+  }
 }
 
 TEMPI_EXPORT int tempi_iid_test(const double *samples, int64_t n,
                                 uint64_t seed, int64_t *counts) {
-  return iid_test(samples, n, seed, counts);
+  try {
+    return iid_test(samples, n, seed, counts);
+  } catch (...) {
+    return set_error(TEMPI_ERR_NOMEM, "IID test of %lld samples", (long long)n);
+  }
 }

@@ -566,7 +566,12 @@ template <class F> int sample_point(F &&iter, const Sampling &sp, IidTime *out) 
     }
     const uint64_t seed = 0x243F6A8885A308D3ull +
                           g_iid_calls.fetch_add(1) * 0x9E3779B97F4A7C15ull;
-    const int iid = iid_test(xs.data(), (int64_t)xs.size(), seed, nullptr);
+    int iid = 0;
+    try {
+      iid = iid_test(xs.data(), (int64_t)xs.size(), seed, nullptr);
+    } catch (...) {
+      iid = set_error(TEMPI_ERR_NOMEM, "IID test of %zu samples", xs.size());
+    }
     if (iid < 0) {
       return iid;
     }
