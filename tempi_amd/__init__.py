@@ -308,6 +308,31 @@ def reset_counters():
     check(lib().tempi_counters_reset())
 
 
+def iid_statistics(samples) -> list:
+    """The eight SP 800-90B section 5.1 statistics of `samples` (see
+    tempi_iid_statistics)."""
+    import numpy as np
+
+    a = np.ascontiguousarray(samples, np.float64)
+    out = (ctypes.c_double * 8)()
+    check(lib().tempi_iid_statistics(a.ctypes.data, a.size, out), "iid_statistics")
+    return list(out)
+
+
+def iid_test(samples, seed: int = 0):
+    """(independent?, counts) of the SP 800-90B section 5.1 permutation test
+    on `samples`: counts[2 k], counts[2 k + 1] = C0, C1 of statistic k (see
+    tempi_iid_test)."""
+    import numpy as np
+
+    a = np.ascontiguousarray(samples, np.float64)
+    c = (ctypes.c_int64 * 16)()
+    r = lib().tempi_iid_test(a.ctypes.data, a.size, ctypes.c_uint64(seed), c)
+    if r < 0:
+        check(r, "iid_test")
+    return r == 1, list(c)
+
+
 def num_variants() -> int:
     return lib().tempi_num_variants()
 

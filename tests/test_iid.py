@@ -180,3 +180,11 @@ def test_trial_count_comes_from_the_environment(monkeypatch):
     assert L.tempi_perf_measure(0, 0.0) == 0
     monkeypatch.delenv("TEMPI_MEASURE_TRIALS")
     assert L.tempi_perf_measure(0, 0.0) == 0
+
+
+def test_python_wrappers():
+    assert ta.iid_statistics([2, 15, 4, 10, 9])[0] == 6.0
+    ok, counts = ta.iid_test([2.0] * 30, seed=1)
+    assert ok and counts[1::2] == [10000] * 8
+    with pytest.raises(ta.TempiError):
+        ta.iid_test([1.0])
